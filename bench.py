@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Throughput of the pseudo-spectral RK2 step on one MI355X.
+
+Workload: 512^3 triply periodic box, FP64, nu = 5e-5, Omega = 2, 3D3C
+forcing on the shell (2, 3] with injection rate 0.1, dt = 1e-3, seeded
+initial spectrum k^4 exp(-2 (k/4)^2) with E = 0.5.  A step is one full
+pass of the reference time-loop body (nlt, step1, nlt, step2, forcing) on
+the state resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32-bulk]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt·steps/s at 512³ RK2 (FP64 vs FP32-bulk), % HBM peak; adjoint step s"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(n, precision):
+    from gturbo3d_amd.params import FORCING_3D3C, PREC_FP32_BULK, PREC_FP64, Params
+    p = Params(nx=n, ny=n, nz=n, nu=5e-5, alpnu=1.0, mu=0.0, alpmu=0.0, dt=1e-3, omega=2.0,
+               rtrunc=2.0 / 3.0, truncate=1, famp=0.0, k1f=2.0, k2f=3.0, variant=FORCING_3D3C,
+               precision=PREC_FP32_BULK if precision == "fp32-bulk" else PREC_FP64)
+    return p
+
+
+def cpu_baseline(p, b, seed, threads):
+    """One oracle step of the same workload on the host cores (the oracle is
+    the CPU restatement of the reference step, OpenMP over pencils)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as ol
+    ol.set_threads(threads)
+    o = ol.Oracle(p)
+    B = [np.ascontiguousarray(f).copy() for f in b]
+    W = [np.ones(p.shape, np.complex128) for _ in range(6)]  # pre-faulted work fields
+    s = ctypes.c_int32(seed)
+    ptrs = [ctypes.c_void_p(x.ctypes.data) for x in B + W]
+    t0 = time.perf_counter()
+    ol.lib().or_step(o.h, *ptrs, ctypes.byref(s), 1)
+    t1 = time.perf_counter()
+    return t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--precision", choices=["fp64", "fp32-bulk"], default="fp64")
+    ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures the single-GPU step only: this build has no multi-GPU path "
+              "(the slab decomposition with its all-to-all is a design in DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from gturbo3d_amd import initial, lib
+
+    n = args.size
+    p0 = workload_params(n, args.precision)
+    # famp = sqrt(0.1 / nf): mean energy input inpe = nf * famp^2 = 0.1
+    nf = lib.forcing_count(p0)[0]
+    p = workload_params(n, args.precision)
+    p.famp = math.sqrt(0.1 / nf)
+    seed0 = 0
+    b = initial.initial_state(p, seed=seed0)
+
+    torch.cuda.init()
+    ctx = lib.Context(p)
+    ctx.set_state(b)
+    seed = seed0
+    seed = ctx.step(args.warmup, seed)
+    ctx.sync()
+
+    ctx.prof_enable(True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    seed = ctx.step(args.steps, seed)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms, launches, bytes_per = ctx.prof_read()
+    pruned = ctx.pruned()
+    out = ctx.get_state()
+    ctx.close()
+    finite = all(np.isfinite(f).all() for f in out)
+
+    t = t1 - t0
+    gps = n ** 3 * args.steps / t
+    ms_step = 1e3 * t / args.steps
+    M = (n // 2 + 1) * n * n
+    S = 16.0 * M
+    dense_bytes = (90.0 if args.precision == "fp64" else 54.0) * S
+    alg_bytes_step = float(sum(bytes_per[q] * launches[q] for q in range(lib.NPASS))) / max(args.steps, 1)
+
+    dom = int(np.argmax(ms))
+    avg_ms = ms[dom] / max(launches[dom], 1)
+    achieved = bytes_per[dom] / (avg_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": lib.PASS_NAMES[dom],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4),
+        "alg_bytes_per_launch": bytes_per[dom],
+    }
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.precision}_{n}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get(lib.PASS_NAMES[dom])
+
+    cpu = None
+    if args.cpu_baseline == "auto":
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        tc = cpu_baseline(p, b, seed0, threads)
+        cpu = {"value": n ** 3 / tc, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
+               "sample": f"1 step of the same {n}^3 FP64 workload, oracle (C + OpenMP) on {threads} threads, "
+                         f"{tc:.1f} s"}
+
+    line = {
+        "metric": METRIC,
+        "value": gps,
+        "unit": "grid-pt·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-bulk (f64 state, IF, RK, reductions)",
+        "data": "synthetic (seeded initial spectrum k^4 exp(-2(k/4)^2), E=0.5; 3D3C forcing seed 0)",
+        "config": {
+            "workload": f"{n}^3 RK2 integrating-factor step, nu=5e-5, Omega=2, 3D3C forcing (2,3] rate 0.1, "
+                        f"dt=1e-3, {args.precision}",
+            "grid": [n, n, n],
+            "pruned": pruned,
+        },
+        "hbm_frac_dense_model": dense_bytes * args.steps / t / (HBM_PEAK_GBS * 1e9),
+        "alg_bytes_per_step": alg_bytes_step,
+        "alg_hbm_frac": alg_bytes_step * args.steps / t / (HBM_PEAK_GBS * 1e9),
+        "pass_ms_per_step": {lib.PASS_NAMES[q]: ms[q] / args.steps for q in range(lib.NPASS)},
+        "state_finite": bool(finite),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

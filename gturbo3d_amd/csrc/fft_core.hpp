@@ -1,0 +1,298 @@
+// fft_core.hpp -- register/LDS Stockham FFT building blocks for CDNA4 (gfx950).
+//
+// A block transforms a batch of pencils of one length N.  Each thread owns
+// a slice t in [0, P) of G pencils; a stage of radix R does the N/R
+// butterflies of every pencil, butterfly j = t + P*m going to thread t.
+// Stage s reads its R inputs at j + r*N/R, twiddles them by
+// exp(sign*2*pi*i*r*k/(Ns*R)) with k = j mod Ns (Ns = product of the
+// earlier radices), does a radix-R DFT in registers and writes the outputs
+// to (j/Ns)*Ns*R + k + r*Ns (the self-sorting Stockham form, Govindaraju
+// et al., SC'08).  The first stage takes its inputs from registers filled
+// by the caller (straight from HBM) and the last stage leaves its outputs
+// in registers (written by the caller straight to HBM), so a transform of
+// S stages makes S-1 round trips through LDS.  The same positions
+// j + r*N/R are used for the first-stage inputs and (with the last radix)
+// for the last-stage outputs, so callers know which elements each thread
+// holds.
+//
+// Twiddles come from a host-built table (long-double sin/cos rounded once);
+// stage s uses entries [off(s) + k*(R-1) + (r-1)] = exp(-2*pi*i*r*k/(Ns*R)),
+// conjugated for the inverse transform.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gt {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+// a * (S*i), S = +1 or -1
+template <int S, typename T>
+__device__ __forceinline__ cx<T> mul_si(cx<T> a) {
+  if constexpr (S > 0) return {-a.y, a.x};
+  else return {a.y, -a.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// ---------------------------------------------------------------------------
+// Radix plans.  Radices are applied in the listed order.
+// ---------------------------------------------------------------------------
+struct PlanInfo {
+  int nst;
+  int r[6];
+};
+
+__host__ __device__ constexpr PlanInfo plan_of(int n) {
+  switch (n) {
+    case 8: return {1, {8, 0, 0, 0, 0, 0}};
+    case 16: return {2, {4, 4, 0, 0, 0, 0}};
+    case 24: return {2, {3, 8, 0, 0, 0, 0}};
+    case 32: return {2, {4, 8, 0, 0, 0, 0}};
+    case 48: return {3, {3, 2, 8, 0, 0, 0}};
+    case 64: return {2, {8, 8, 0, 0, 0, 0}};
+    case 96: return {3, {3, 4, 8, 0, 0, 0}};
+    case 128: return {3, {2, 8, 8, 0, 0, 0}};
+    case 192: return {3, {3, 8, 8, 0, 0, 0}};
+    case 256: return {3, {4, 8, 8, 0, 0, 0}};
+    case 384: return {4, {3, 2, 8, 8, 0, 0}};
+    case 512: return {3, {8, 8, 8, 0, 0, 0}};
+    case 768: return {4, {3, 4, 8, 8, 0, 0}};
+    case 1024: return {4, {2, 8, 8, 8, 0, 0}};
+    default: return {0, {0, 0, 0, 0, 0, 0}};
+  }
+}
+
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+template <int N>
+struct Plan {
+  static constexpr PlanInfo I = plan_of(N);
+  static_assert(I.nst > 0, "unsupported FFT length");
+  static constexpr int S = I.nst;
+  static constexpr int radix(int s) { return I.r[s]; }
+  static constexpr int ns(int s) {
+    int v = 1;
+    for (int q = 0; q < s; q++) v *= I.r[q];
+    return v;
+  }
+  // offset of stage s in the twiddle table
+  static constexpr int twoff(int s) {
+    int o = 0;
+    for (int q = 0; q < s; q++) o += ns(q) * (I.r[q] - 1);
+    return o;
+  }
+  static constexpr int twsize() { return twoff(S); }
+};
+
+// ---------------------------------------------------------------------------
+// Radix-R DFTs in registers, y[q] = sum_r v[r] exp(S*2*pi*i*r*q/R).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct K {
+  static constexpr T h2 = T(0.70710678118654752440084436210484904);  // sqrt(2)/2
+  static constexpr T h3 = T(0.86602540378443864676372317075293618);  // sqrt(3)/2
+};
+
+template <int S, typename T>
+__device__ __forceinline__ void dft2(cx<T>* v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft3(cx<T>* v) {
+  cx<T> a = v[0], b = v[1], c = v[2];
+  cx<T> s = cadd(b, c);
+  cx<T> d = csub(b, c);
+  v[0] = cadd(a, s);
+  cx<T> m = {a.x - T(0.5) * s.x, a.y - T(0.5) * s.y};
+  cx<T> e = mul_si<S>(cscale(d, K<T>::h3));
+  v[1] = cadd(m, e);
+  v[2] = csub(m, e);
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft4(cx<T>* v) {
+  cx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  cx<T> c = cadd(v[1], v[3]), d = mul_si<S>(csub(v[1], v[3]));
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+  // decimation in frequency: a[n] = v[n] + v[n+4], b[n] = (v[n] - v[n+4]) w^n
+  cx<T> a[4], b[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    a[n] = cadd(v[n], v[n + 4]);
+    b[n] = csub(v[n], v[n + 4]);
+  }
+  const T h = K<T>::h2;
+  // w = (1 + S i)/sqrt2 ; w^2 = S i ; w^3 = (-1 + S i)/sqrt2
+  b[1] = {(b[1].x - T(S) * b[1].y) * h, (b[1].y + T(S) * b[1].x) * h};
+  b[2] = mul_si<S>(b[2]);
+  b[3] = {(-b[3].x - T(S) * b[3].y) * h, (-b[3].y + T(S) * b[3].x) * h};
+  dft4<S>(a);
+  dft4<S>(b);
+  v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
+  v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
+}
+
+template <int R, int S, typename T>
+__device__ __forceinline__ void dft(cx<T>* v) {
+  if constexpr (R == 2) dft2<S>(v);
+  else if constexpr (R == 3) dft3<S>(v);
+  else if constexpr (R == 4) dft4<S>(v);
+  else if constexpr (R == 8) dft8<S>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ---------------------------------------------------------------------------
+// LDS layouts.  off(g, n) is the complex offset of element n of the g-th
+// pencil the thread works on (p = pencil index within the block).
+// ---------------------------------------------------------------------------
+
+// T pencils interleaved: element n of pencil p at n*T + p.  Lanes with
+// consecutive p touch consecutive 16-byte words, so a group of 8 lanes
+// covers one 128-byte row and every access pattern is conflict-free.
+template <int T>
+struct LayInterleaved {
+  int p;
+  __device__ __forceinline__ int off(int /*g*/, int n) const { return n * T + p; }
+};
+
+// Pencils stored one after another with one pad word every 8 (stride
+// N + N/8): used when lanes run along n (contiguous rows in HBM).  The pad
+// spreads the stride-R writes of the early Stockham stages over the banks.
+template <int N>
+struct LayRows {
+  int s0;     // first pencil of the thread
+  int sstep;  // pencil step between the thread's G pencils
+  static constexpr int STR = N + N / 8;
+  __device__ __forceinline__ int off(int g, int n) const { return (s0 + g * sstep) * STR + n + (n >> 3); }
+};
+
+// ---------------------------------------------------------------------------
+// The staged transform.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int P, int G, int SIGN, class L>
+struct Fft {
+  using PL = Plan<N>;
+  static constexpr int S = PL::S;
+  static constexpr int R0 = PL::radix(0);
+  static constexpr int RL = PL::radix(S - 1);
+  static constexpr int NR0 = N / R0;
+  static constexpr int NRL = N / RL;
+  static constexpr int M0 = cdiv(NR0, P);
+  static constexpr int ML = cdiv(NRL, P);
+  static constexpr int E0 = M0 * R0;  // first-stage values per pencil per thread
+  static constexpr int EL = ML * RL;  // last-stage values per pencil per thread
+
+  // element index held in slot (m, r) of the first / last stage
+  __device__ static __forceinline__ int in_pos(int t, int m, int r) { return t + P * m + r * NR0; }
+  __device__ static __forceinline__ bool in_ok(int t, int m) { return (NR0 % P == 0) || (t + P * m < NR0); }
+  __device__ static __forceinline__ int out_pos(int t, int m, int r) { return t + P * m + r * NRL; }
+  __device__ static __forceinline__ bool out_ok(int t, int m) { return (NRL % P == 0) || (t + P * m < NRL); }
+
+  template <int s>
+  __device__ static __forceinline__ void stage(const L& lay, cx<T>* lds, const cx<T>* __restrict__ tw, int t,
+                                               cx<T> (&in)[G][E0], cx<T> (&out)[G][EL]) {
+    constexpr int R = PL::radix(s);
+    constexpr int Ns = PL::ns(s);
+    constexpr int NR = N / R;
+    constexpr int M = cdiv(NR, P);
+    constexpr bool FULL = (NR % P == 0);
+    cx<T> v[G][M * R];
+    if constexpr (s == 0) {
+#pragma unroll
+      for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int q = 0; q < M * R; q++) v[g][q] = in[g][q];
+    } else {
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int j = t + P * m;
+        if (FULL || j < NR) {
+#pragma unroll
+          for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < R; r++) v[g][m * R + r] = lds[lay.off(g, j + r * NR)];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const int j = t + P * m;
+      if (FULL || j < NR) {
+        if constexpr (s > 0) {
+          const int k = j % Ns;
+          const cx<T>* w = tw + PL::twoff(s) + k * (R - 1);
+          cx<T> wr[R - 1];
+#pragma unroll
+          for (int r = 1; r < R; r++) wr[r - 1] = w[r - 1];
+#pragma unroll
+          for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+              if constexpr (SIGN < 0) v[g][m * R + r] = cmul(v[g][m * R + r], wr[r - 1]);
+              else v[g][m * R + r] = cmulc(v[g][m * R + r], wr[r - 1]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) dft<R, SIGN>(&v[g][m * R]);
+      }
+    }
+    if constexpr (s == S - 1) {
+#pragma unroll
+      for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int q = 0; q < M * R; q++) out[g][q] = v[g][q];
+    } else {
+      __syncthreads();  // every read of the previous stage (or previous use) is done
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int j = t + P * m;
+        if (FULL || j < NR) {
+          const int k = j % Ns;
+          const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+          for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lay.off(g, d + r * Ns)] = v[g][m * R + r];
+        }
+      }
+      __syncthreads();
+      stage<s + 1>(lay, lds, tw, t, in, out);
+    }
+  }
+
+  // Transform the values in `in` (first-stage layout) into `out` (last-stage
+  // layout).  All threads of the block must call it (it has barriers).
+  __device__ static __forceinline__ void run(const L& lay, cx<T>* lds, const cx<T>* __restrict__ tw, int t,
+                                             cx<T> (&in)[G][E0], cx<T> (&out)[G][EL]) {
+    stage<0>(lay, lds, tw, t, in, out);
+  }
+};
+
+}  // namespace gt

@@ -1,0 +1,845 @@
+// gt_api.cpp -- host side of the library: the context, the reference's
+// host tables (defk, iniforcing), the forcing LCG, the truncation extents
+// and the C ABI declared in include/gturbo3d.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gturbo3d.h"
+#include "gt_config.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+hipError_t launch_zinv(int prec, int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s);
+hipError_t launch_zfwd(int prec, int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
+                       hipStream_t s);
+hipError_t launch_yfft(int prec, int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s);
+hipError_t launch_xfused(int prec, const KP& kp, void* work, const void* tw, hipStream_t s);
+hipError_t launch_xc2r(const KP& kp, const void* work, double* phys, const void* tw, hipStream_t s);
+hipError_t launch_xr2c(const KP& kp, const double* phys, void* work, const void* tw, hipStream_t s);
+hipError_t launch_forcing(void* b, long long fs, const FEnt* ent, int n, bool sequential, hipStream_t s);
+enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
+}  // namespace gt
+
+using namespace gt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(GT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int NFORC = 2048;  // src/modules.f90:39
+
+struct Forced {
+  int i, j, k;  // 1-based (kx, ky, kz) indices as in the reference
+  double ff;
+};
+
+}  // namespace
+
+struct gt_ctx {
+  gt_params p{};
+  int nx = 0, ny = 0, nz = 0, nxh = 0, nxp = 0;
+  long long fs = 0;
+  int prec = 0;
+  hipStream_t stream = nullptr;
+
+  // host tables (src/defk.f90)
+  std::vector<double> fkx, fky, fkz, fkxs, fkys, fkzs, fkxn, fkyn, fkzn;
+  std::vector<int> jc, kc;
+  double fkxsmax = 0, fkysmax = 0, fkzsmax = 0, rtrunc2 = 0, sdt = 0, dt2 = 0;
+  std::vector<Forced> forced;
+  double inpe = 0, inpz = 0;
+  bool force_seq = false;
+
+  // truncation extents
+  int kxcut = 0;
+  std::vector<int16_t> kyext, kzext;
+  bool pruned = false;
+
+  // device memory
+  cx<double>* b = nullptr;   // 3 fields
+  cx<double>* bh = nullptr;  // 3 fields
+  void* work = nullptr;      // 6 fields of the work precision
+  double* dtab = nullptr;    // fkx fky fkz fkxn fkyn fkzn
+  int16_t* dkyext = nullptr;
+  int16_t* dkzext = nullptr;
+  void* tw64[3] = {nullptr, nullptr, nullptr};  // x, y, z
+  void* tw32[3] = {nullptr, nullptr, nullptr};
+  FEnt* dforce = nullptr;
+  size_t dforce_cap = 0;
+  FEnt* hforce = nullptr;  // pinned staging
+  size_t hforce_cap = 0;
+  hipEvent_t hforce_done = nullptr;
+  bool hforce_pending = false;
+
+  // profiling
+  bool prof = false;
+  struct Rec {
+    int pass;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[NPASS] = {0};
+  long long prof_n[NPASS] = {0};
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+
+// defk (src/defk.f90:13-54, 88-91)
+void build_wavenumbers(gt_ctx* c) {
+  const gt_params& p = c->p;
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = c->nxh;
+  c->jc.assign(ny, 1);
+  for (int j = 2; j <= ny; j++) c->jc[j - 1] = ny + 2 - j;
+  c->kc.assign(nz, 1);
+  for (int k = 2; k <= nz; k++) c->kc[k - 1] = nz + 2 - k;
+  const double dkx = 2.0 * 3.14159265358979323846 / p.xlx;
+  const double dky = 2.0 * 3.14159265358979323846 / p.xly;
+  const double dkz = 2.0 * 3.14159265358979323846 / p.xlz;
+  c->fkx.resize(nxh);
+  c->fky.resize(ny);
+  c->fkz.resize(nz);
+  for (int i = 1; i <= nxh; i++) c->fkx[i - 1] = (double)(i - 1) * dkx;
+  for (int j = 1; j <= ny / 2 + 1; j++) c->fky[j - 1] = (double)(j - 1) * dky;
+  for (int j = ny / 2 + 2; j <= ny; j++) c->fky[j - 1] = (double)(j - ny - 1) * dky;
+  for (int k = 1; k <= nz / 2 + 1; k++) c->fkz[k - 1] = (double)(k - 1) * dkz;
+  for (int k = nz / 2 + 2; k <= nz; k++) c->fkz[k - 1] = (double)(k - nz - 1) * dkz;
+  c->fkxs.resize(nxh);
+  c->fkys.resize(ny);
+  c->fkzs.resize(nz);
+  for (int i = 0; i < nxh; i++) c->fkxs[i] = c->fkx[i] * c->fkx[i];
+  for (int j = 0; j < ny; j++) c->fkys[j] = c->fky[j] * c->fky[j];
+  for (int k = 0; k < nz; k++) c->fkzs[k] = c->fkz[k] * c->fkz[k];
+  c->fkxsmax = c->fkxs[nxh - 1];
+  c->fkysmax = c->fkys[ny / 2];
+  c->fkzsmax = c->fkzs[nz / 2];
+  c->rtrunc2 = p.rtrunc * p.rtrunc;
+  // the three ratios of the trunc test (src/trunc.f90:14-16)
+  c->fkxn.resize(nxh);
+  c->fkyn.resize(ny);
+  c->fkzn.resize(nz);
+  for (int i = 0; i < nxh; i++) c->fkxn[i] = c->fkxs[i] / c->fkxsmax;
+  for (int j = 0; j < ny; j++) c->fkyn[j] = c->fkys[j] / c->fkysmax;
+  for (int k = 0; k < nz; k++) c->fkzn[k] = c->fkzs[k] / c->fkzsmax;
+  (void)nx;
+}
+
+bool trunc_masked(const gt_ctx* c, int i, int j, int k) {
+  // src/trunc.f90:14-17 (rtrunc squared at src/defk.f90:91)
+  const double k3 = c->fkzn[k];
+  const double k2 = k3 + c->fkyn[j];
+  const double k1 = k2 + c->fkxn[i];
+  return k1 > c->rtrunc2;
+}
+
+// iniforcing: one candidate mode (src/iniforcing_3d3c.f90:26-40)
+int forced_try(gt_ctx* c, int i, int j, int k) {
+  const double k2 = c->fkxs[i - 1] + c->fkys[j - 1] + c->fkzs[k - 1];
+  const double kk = std::sqrt(k2);
+  const double k2n = c->fkxs[i - 1] / c->fkxsmax + c->fkys[j - 1] / c->fkysmax + c->fkzs[k - 1] / c->fkzsmax;
+  if (k2n <= c->rtrunc2 && k2 <= c->p.k2f * c->p.k2f && k2 > c->p.k1f * c->p.k1f) {
+    if ((int)c->forced.size() >= NFORC) return -1;
+    const double ff = 1.0 / kk;
+    c->forced.push_back({i, j, k, ff});
+    if (c->p.variant == GT_FORCING_3D3C) {
+      const double a = c->p.famp * ff;
+      c->inpz = c->inpz + k2 * k2 * (a * a);
+      c->inpe = c->inpe + k2 * (a * a);
+    }
+  }
+  return 0;
+}
+
+// iniforcing in the reference enumeration order, which fixes the order of
+// the random draws (src/iniforcing_3d3c.f90:22-184, src/iniforcing_2d2c.f90:12-68)
+int build_forcing(gt_ctx* c) {
+  c->forced.clear();
+  c->inpe = c->inpz = 0.0;
+  const int nx2 = c->nx / 2, nx2p1 = nx2 + 1, ny = c->ny, nz = c->nz, ny2 = ny / 2, nz2 = nz / 2;
+#define TRY(i, j, k) \
+  if (forced_try(c, i, j, k)) return -1
+  if (c->p.variant == GT_FORCING_3D3C) {
+    for (int pl = 0; pl < 2; pl++) {  // kx = 0 plane, then kx = Nyquist plane
+      if (pl == 1) {
+        for (int i = 2; i <= nx2; i++)
+          for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++) TRY(i, j, k);
+      }
+      const int i = (pl == 0) ? 1 : nx2p1;
+      for (int k = 2; k <= nz2; k++) TRY(i, 1, k);
+      for (int j = 2; j <= ny2; j++)
+        for (int k = 1; k <= nz; k++) TRY(i, j, k);
+      for (int k = 2; k <= nz2; k++) TRY(i, ny2 + 1, k);
+    }
+  } else {
+    const int k = 1;
+    for (int j = 2; j <= ny2; j++) TRY(1, j, k);
+    for (int i = 2; i <= nx2; i++)
+      for (int j = 1; j <= ny; j++) TRY(i, j, k);
+    for (int j = 2; j <= ny2; j++) TRY(nx2p1, j, k);
+    for (const Forced& f : c->forced) {
+      const double k2 = c->fkxs[f.i - 1] + c->fkys[f.j - 1] + c->fkzs[f.k - 1];
+      const double a = c->p.famp * f.ff;
+      c->inpe = c->inpe + k2 * (a * a);
+      c->inpz = c->inpz + k2 * k2 * (a * a);
+    }
+  }
+#undef TRY
+  return 0;
+}
+
+long long elem_off(const gt_ctx* c, int i0, int j0, int k0) {  // 0-based (kx, ky, kz)
+  return ((long long)k0 * c->ny + j0) * c->nxp + i0;
+}
+
+// Extents of the modes that survive the truncation, widened to cover the
+// forced modes and their conjugates.
+void build_extents(gt_ctx* c) {
+  const int nxh = c->nxh, nxp = c->nxp, ny = c->ny, nz = c->nz;
+  c->kzext.assign((size_t)ny * nxp, (int16_t)-1);
+  c->kyext.assign(nxp, (int16_t)-1);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nxh; i++) {
+      int K = -1;
+      for (int m = 0; m <= nz / 2; m++) {
+        if (trunc_masked(c, i, j, m)) break;
+        K = m;
+      }
+      c->kzext[(size_t)j * nxp + i] = (int16_t)K;
+    }
+  auto widen = [&](int i0, int j0, int k0) {
+    const int mz = std::min(k0, nz - k0);
+    int16_t& K = c->kzext[(size_t)j0 * nxp + i0];
+    if (K < mz) K = (int16_t)mz;
+  };
+  for (const Forced& f : c->forced) {
+    widen(f.i - 1, f.j - 1, f.k - 1);
+    widen(f.i - 1, c->jc[f.j - 1] - 1, c->kc[f.k - 1] - 1);
+  }
+  // a column is alive iff |ky| <= kyext[kx]; every column inside is kept
+  for (int i = 0; i < nxh; i++) {
+    int Ky = -1;
+    for (int j = 0; j < ny; j++)
+      if (c->kzext[(size_t)j * nxp + i] >= 0) Ky = std::max(Ky, std::min(j, ny - j));
+    c->kyext[i] = (int16_t)Ky;
+    for (int j = 0; j < ny; j++)
+      if (std::min(j, ny - j) <= Ky && c->kzext[(size_t)j * nxp + i] < 0) c->kzext[(size_t)j * nxp + i] = 0;
+  }
+  c->kxcut = -1;
+  for (int i = 0; i < nxh; i++)
+    if (c->kyext[i] >= 0) c->kxcut = i;
+  if (c->kxcut < 0) c->kxcut = 0;
+}
+
+// Stockham twiddles exp(-2 pi i r k/(Ns R)) per stage (fft_core.hpp)
+template <typename T>
+std::vector<T> twiddles(int n) {
+  const PlanInfo pi = plan_of(n);
+  std::vector<T> tw;
+  int Ns = 1;
+  for (int s = 0; s < pi.nst; s++) {
+    const int R = pi.r[s];
+    for (int k = 0; k < Ns; k++)
+      for (int r = 1; r < R; r++) {
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)(r * k) /
+                              (long double)(Ns * R);
+        tw.push_back((T)cosl(a));
+        tw.push_back((T)sinl(a));
+      }
+    Ns *= R;
+  }
+  if (tw.empty()) tw.push_back(T(0));
+  return tw;
+}
+
+KP make_kp(const gt_ctx* c, bool dense) {
+  KP kp{};
+  kp.nx = c->nx;
+  kp.ny = c->ny;
+  kp.nz = c->nz;
+  kp.nxh = c->nxh;
+  kp.nxp = c->nxp;
+  kp.fs = c->fs;
+  kp.fkx = c->dtab;
+  kp.fky = c->dtab + c->nxh;
+  kp.fkz = c->dtab + c->nxh + c->ny;
+  kp.fkxn = c->dtab + c->nxh + c->ny + c->nz;
+  kp.fkyn = kp.fkxn + c->nxh;
+  kp.fkzn = kp.fkyn + c->ny;
+  kp.rtrunc2 = c->rtrunc2;
+  kp.truncate = c->p.truncate;
+  const bool pr = c->pruned && !dense;
+  kp.pruned = pr ? 1 : 0;
+  kp.kxcut = pr ? c->kxcut : c->nxh - 1;
+  kp.kyext = c->dkyext;
+  kp.kzext = c->dkzext;
+  kp.omega = c->p.omega;
+  kp.dt = c->p.dt;
+  kp.dt2 = c->dt2;
+  kp.nu = c->p.nu;
+  kp.mu = c->p.mu;
+  kp.alpnu = c->p.alpnu;
+  kp.alpmu = c->p.alpmu;
+  kp.scale = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+  kp.visc_simple = (c->p.alpnu == 1.0 && c->p.mu == 0.0) ? 1 : 0;
+  return kp;
+}
+
+const void* twp(const gt_ctx* c, int axis) { return c->prec ? c->tw32[axis] : c->tw64[axis]; }
+
+hipEvent_t pool_get(gt_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct ProfScope {
+  gt_ctx* c;
+  int pass;
+  hipEvent_t a = nullptr;
+  ProfScope(gt_ctx* c_, int pass_) : c(c_), pass(pass_) {
+    if (c->prof) {
+      a = pool_get(c);
+      if (a) hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof && a) {
+      hipEvent_t b = pool_get(c);
+      if (b) {
+        hipEventRecord(b, c->stream);
+        c->recs.push_back({pass, a, b});
+      }
+    }
+  }
+};
+
+#define LAUNCH(pass, expr)                                                                      \
+  do {                                                                                          \
+    ProfScope ps_(c, pass);                                                                     \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail(GT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// One evaluation of N(a; b) through passes A-D, then the pass-E epilogue.
+int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out) {
+  const int pr = c->prec;
+  LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), c->stream));
+  LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kp, c->work, twp(c, 1), c->stream));
+  LAUNCH(PASS_C, launch_xfused(pr, kp, c->work, twp(c, 0), c->stream));
+  LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), c->stream));
+  const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
+  LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), c->stream));
+  return GT_OK;
+}
+
+// forcing increments of one step (src/forcing_3d3c.f90:26-72,
+// src/forcing_2d2c.f90:120-135); appends entries and advances the seed
+void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out) {
+  auto rann = [](int32_t* s) -> float {  // src/forcing_3d3c.f90:82-93
+    const uint32_t mask = 0x7FFFFFFFu;
+    uint32_t v = (69069u * (uint32_t)(*s) + 1u) & mask;
+    *s = (int32_t)v;
+    return (float)((double)v / (double)mask);
+  };
+  const int nxh = c->nxh;
+  const double sf = c->sdt * c->p.famp;
+  size_t o = 0;
+  for (const Forced& fm : c->forced) {
+    const int i = fm.i, j = fm.j, k = fm.k;
+    const long long q = elem_off(c, i - 1, j - 1, k - 1);
+    const long long qc = elem_off(c, i - 1, c->jc[j - 1] - 1, c->kc[k - 1] - 1);
+    if (c->p.variant == GT_FORCING_2D2C) {
+      const double pi2 = 2.0 * 3.14159265358979;
+      const double te = pi2 * (double)rann(seed);
+      const double er = sf * std::cos(te), ei = sf * std::sin(te);
+      const double kr = er * fm.ff, ki = ei * fm.ff;
+      out[o++] = {q, {0.0, 0.0, 0.0, 0.0, kr, ki}};
+      if (i == 1 || i == nxh) out[o++] = {qc, {0.0, 0.0, 0.0, 0.0, kr, -ki}};
+      continue;
+    }
+    const double pi2 = 2.0 * 3.14159265358979323846;
+    double te = pi2 * (double)rann(seed);
+    const double er = sf * std::cos(te), ei = sf * std::sin(te);
+    te = pi2 * (double)rann(seed);
+    const double cte = std::cos(te), ste = std::sin(te);
+    const double kr = er * fm.ff, ki = ei * fm.ff;
+    if (i == 1 && j == 1) {
+      out[o++] = {q, {kr * cte, ki * cte, kr * ste, ki * ste, 0.0, 0.0}};
+      out[o++] = {qc, {kr * cte, (-ki) * cte, kr * ste, (-ki) * ste, 0.0, 0.0}};
+    } else {
+      double k2 = c->fkxs[i - 1] + c->fkys[j - 1];
+      const double kxy = std::sqrt(k2);
+      k2 = k2 + c->fkzs[k - 1];
+      const double kk = std::sqrt(k2);
+      const double c1 = c->fkz[k - 1] / kk, s1 = kxy / kk;
+      const double c2 = c->fkx[i - 1] / kxy, s2 = c->fky[j - 1] / kxy;
+      const double f1 = cte * c2 * c1 - ste * s2;
+      const double f2 = cte * s2 * c1 + ste * c2;
+      // bz - effek*cte*s1 == bz + (-(effek*cte*s1)) exactly
+      out[o++] = {q, {kr * f1, ki * f1, kr * f2, ki * f2, -(kr * cte * s1), -(ki * cte * s1)}};
+      if (i == 1 || i == nxh)
+        out[o++] = {qc, {kr * f1, (-ki) * f1, kr * f2, (-ki) * f2, -(kr * cte * s1), -((-ki) * cte * s1)}};
+    }
+  }
+}
+
+size_t entries_per_step(const gt_ctx* c) {
+  size_t n = 0;
+  for (const Forced& f : c->forced) {
+    n += 1;
+    const bool conj = (c->p.variant == GT_FORCING_2D2C) ? (f.i == 1 || f.i == c->nxh)
+                                                          : ((f.i == 1 && f.j == 1) || f.i == 1 || f.i == c->nxh);
+    if (conj) n += 1;
+  }
+  return n;
+}
+
+void free_ctx(gt_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) hipEventDestroy(e);
+  if (c->hforce_done) hipEventDestroy(c->hforce_done);
+  hipFree(c->b);
+  hipFree(c->bh);
+  hipFree(c->work);
+  hipFree(c->dtab);
+  hipFree(c->dkyext);
+  hipFree(c->dkzext);
+  for (int a = 0; a < 3; a++) {
+    hipFree(c->tw64[a]);
+    hipFree(c->tw32[a]);
+  }
+  hipFree(c->dforce);
+  if (c->hforce) hipHostFree(c->hforce);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int ensure_force_cap(gt_ctx* c, size_t n) {
+  if (n <= c->dforce_cap && n <= c->hforce_cap) return GT_OK;
+  if (c->hforce_pending) {
+    HIPCHK(hipEventSynchronize(c->hforce_done));
+    c->hforce_pending = false;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  size_t cap = std::max<size_t>(n, 2 * std::max(c->dforce_cap, (size_t)64));
+  hipFree(c->dforce);
+  c->dforce = nullptr;
+  if (c->hforce) hipHostFree(c->hforce);
+  c->hforce = nullptr;
+  c->dforce_cap = c->hforce_cap = 0;
+  HIPCHK(hipMalloc(&c->dforce, cap * sizeof(FEnt)));
+  HIPCHK(hipHostMalloc((void**)&c->hforce, cap * sizeof(FEnt), hipHostMallocDefault));
+  c->dforce_cap = c->hforce_cap = cap;
+  return GT_OK;
+}
+
+// Set pruned mode and make bh vanish outside the kept extents.
+int set_mode(gt_ctx* c, bool pruned) {
+  c->pruned = pruned;
+  HIPCHK(hipMemsetAsync(c->bh, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
+  return GT_OK;
+}
+
+int check_ctx(gt_ctx* c) {
+  if (!c) return fail(GT_EINVAL, "null context");
+  return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gt_last_error(void) { return g_err.c_str(); }
+
+int gt_create(const gt_params* p, gt_ctx** out) {
+  if (!p || !out) return fail(GT_EINVAL, "gt_create: null argument");
+  *out = nullptr;
+  for (int n : {p->nx, p->ny, p->nz})
+    if (!supported_n(n))
+      return fail(GT_ESIZE, "gt_create: grid size " + std::to_string(n) +
+                                " not compiled (supported: 16 24 32 48 64 96 128 192 256 384 512 768 1024)");
+  if (p->variant != GT_FORCING_3D3C && p->variant != GT_FORCING_2D2C)
+    return fail(GT_EINVAL, "gt_create: variant must be 3 (3D3C) or 2 (2D2C)");
+  if (p->dt2_mode != GT_DT2_REFERENCE && p->dt2_mode != GT_DT2_MIDPOINT)
+    return fail(GT_EINVAL, "gt_create: dt2_mode must be 0 or 1");
+  if (p->precision != GT_PREC_FP64 && p->precision != GT_PREC_FP32_BULK)
+    return fail(GT_EINVAL, "gt_create: precision must be 0 or 1");
+  gt_ctx* c = new (std::nothrow) gt_ctx();
+  if (!c) return fail(GT_ENOMEM, "gt_create: out of host memory");
+  c->p = *p;
+  c->nx = p->nx;
+  c->ny = p->ny;
+  c->nz = p->nz;
+  c->nxh = p->nx / 2 + 1;
+  c->nxp = (c->nxh + 15) / 16 * 16;
+  c->fs = (long long)c->nz * c->ny * c->nxp;
+  c->prec = p->precision;
+  c->sdt = std::sqrt(p->dt);                              // src/inieuler.f90:67
+  c->dt2 = (p->dt2_mode == GT_DT2_MIDPOINT) ? 0.5 * p->dt : 0.0;
+  build_wavenumbers(c);
+  if (build_forcing(c)) {
+    delete c;
+    return fail(GT_ENFORC, "gt_create: NFORC too small ! (more than 2048 forced modes)");
+  }
+  build_extents(c);
+  {
+    std::vector<long long> offs;
+    for (const Forced& f : c->forced) {
+      offs.push_back(elem_off(c, f.i - 1, f.j - 1, f.k - 1));
+      const bool conj = (p->variant == GT_FORCING_2D2C) ? (f.i == 1 || f.i == c->nxh)
+                                                        : ((f.i == 1 && f.j == 1) || f.i == 1 || f.i == c->nxh);
+      if (conj) offs.push_back(elem_off(c, f.i - 1, c->jc[f.j - 1] - 1, c->kc[f.k - 1] - 1));
+    }
+    std::sort(offs.begin(), offs.end());
+    c->force_seq = std::adjacent_find(offs.begin(), offs.end()) != offs.end();
+  }
+
+  int rc = GT_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == GT_OK) rc = fail(e == hipErrorOutOfMemory ? GT_ENOMEM : GT_EHIP,
+                                                  std::string("gt_create: ") + what + ": " + hipGetErrorString(e));
+  };
+  chk(hipStreamCreate(&c->stream), "hipStreamCreate");
+  const size_t cw = c->prec ? sizeof(cx<float>) : sizeof(cx<double>);
+  if (rc == GT_OK) chk(hipMalloc(&c->b, sizeof(cx<double>) * 3 * c->fs), "state");
+  if (rc == GT_OK) chk(hipMalloc(&c->bh, sizeof(cx<double>) * 3 * c->fs), "half-step state");
+  if (rc == GT_OK) chk(hipMalloc(&c->work, cw * 6 * c->fs), "work fields");
+  std::vector<double> tab;
+  for (auto* v : {&c->fkx, &c->fky, &c->fkz, &c->fkxn, &c->fkyn, &c->fkzn}) tab.insert(tab.end(), v->begin(), v->end());
+  if (rc == GT_OK) chk(hipMalloc(&c->dtab, sizeof(double) * tab.size()), "tables");
+  if (rc == GT_OK) chk(hipMemcpy(c->dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "tables");
+  if (rc == GT_OK) chk(hipMalloc(&c->dkyext, sizeof(int16_t) * c->kyext.size()), "extents");
+  if (rc == GT_OK)
+    chk(hipMemcpy(c->dkyext, c->kyext.data(), sizeof(int16_t) * c->kyext.size(), hipMemcpyHostToDevice), "extents");
+  if (rc == GT_OK) chk(hipMalloc(&c->dkzext, sizeof(int16_t) * c->kzext.size()), "extents");
+  if (rc == GT_OK)
+    chk(hipMemcpy(c->dkzext, c->kzext.data(), sizeof(int16_t) * c->kzext.size(), hipMemcpyHostToDevice), "extents");
+  const int ns[3] = {c->nx, c->ny, c->nz};
+  for (int a = 0; a < 3 && rc == GT_OK; a++) {
+    std::vector<double> t64 = twiddles<double>(ns[a]);
+    chk(hipMalloc(&c->tw64[a], sizeof(double) * t64.size()), "twiddles");
+    if (rc == GT_OK) chk(hipMemcpy(c->tw64[a], t64.data(), sizeof(double) * t64.size(), hipMemcpyHostToDevice), "twiddles");
+    if (c->prec && rc == GT_OK) {
+      std::vector<float> t32 = twiddles<float>(ns[a]);
+      chk(hipMalloc(&c->tw32[a], sizeof(float) * t32.size()), "twiddles");
+      if (rc == GT_OK)
+        chk(hipMemcpy(c->tw32[a], t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "twiddles");
+    }
+  }
+  if (rc == GT_OK) chk(hipEventCreateWithFlags(&c->hforce_done, hipEventDisableTiming), "event");
+  if (rc == GT_OK) chk(hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream), "memset");
+  if (rc == GT_OK) chk(hipMemsetAsync(c->work, 0, cw * 6 * c->fs, c->stream), "memset");
+  if (rc == GT_OK) rc = set_mode(c, c->p.truncate == 1);
+  if (rc == GT_OK) chk(hipStreamSynchronize(c->stream), "sync");
+  if (rc != GT_OK) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return GT_OK;
+}
+
+int gt_destroy(gt_ctx* c) {
+  if (!c) return GT_OK;
+  free_ctx(c);
+  return GT_OK;
+}
+
+int gt_set_state(gt_ctx* c, const double* bx, const double* by, const double* bz) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!bx || !by || !bz) return fail(GT_EINVAL, "gt_set_state: null field");
+  const double* src[3] = {bx, by, bz};
+  // pruned only if every field vanishes outside the kept extents
+  bool inside = (c->p.truncate == 1);
+  for (int f = 0; f < 3 && inside; f++) {
+    const double* s = src[f];
+    for (int k = 0; k < c->nz && inside; k++)
+      for (int j = 0; j < c->ny && inside; j++) {
+        const int K = c->kzext[(size_t)j * c->nxp];  // column test below is per kx
+        (void)K;
+        const int mk = std::min(k, c->nz - k);
+        for (int i = 0; i < c->nxh; i++) {
+          const int Kc = c->kzext[(size_t)j * c->nxp + i];
+          if (mk <= Kc) continue;
+          const size_t q = (((size_t)k * c->ny + j) * c->nxh + i) * 2;
+          if (s[q] != 0.0 || s[q + 1] != 0.0) {
+            inside = false;
+            break;
+          }
+        }
+      }
+  }
+  HIPCHK(hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpy2DAsync(c->b + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
+                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->stream));
+  if (int rc = set_mode(c, inside)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GT_OK;
+}
+
+int gt_get_state(gt_ctx* c, double* bx, double* by, double* bz) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!bx || !by || !bz) return fail(GT_EINVAL, "gt_get_state: null field");
+  double* dst[3] = {bx, by, bz};
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, c->b + f * c->fs, sizeof(cx<double>) * c->nxp,
+                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GT_OK;
+}
+
+int gt_step(gt_ctx* c, int nsteps, int32_t* seed) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!seed) return fail(GT_EINVAL, "gt_step: null seed");
+  if (nsteps < 0) return fail(GT_EINVAL, "gt_step: negative step count");
+  if (nsteps == 0) return GT_OK;
+  const size_t per = entries_per_step(c);
+  if (per > 0) {
+    if (int rc = ensure_force_cap(c, per * (size_t)nsteps)) return rc;
+    if (c->hforce_pending) {
+      HIPCHK(hipEventSynchronize(c->hforce_done));
+      c->hforce_pending = false;
+    }
+    for (int s = 0; s < nsteps; s++) forcing_entries(c, seed, c->hforce + per * s);
+    HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * per * nsteps, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->hforce_done, c->stream));
+    c->hforce_pending = true;
+  } else {
+    for (int s = 0; s < nsteps; s++) forcing_entries(c, seed, nullptr);
+  }
+  const KP kp = make_kp(c, false);
+  for (int s = 0; s < nsteps; s++) {
+    // stage 1: N1 = nlt(b; b), bh = emk (b + dt2 N1)      (src/main.f90:124-141)
+    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh)) return rc;
+    // stage 2: N2 = nlt(bh; b), b = emk (emk b + dt N2)   (src/main.f90:144-146)
+    if (int rc = nonlinear_stage(c, kp, c->bh, EPI_STEP2, c->b)) return rc;
+    // forcing (src/main.f90:149)
+    if (per > 0) LAUNCH(PASS_F, launch_forcing(c->b, c->fs, c->dforce + per * s, (int)per, c->force_seq, c->stream));
+  }
+  return GT_OK;
+}
+
+int gt_nonlinear(gt_ctx* c, double* nx, double* ny, double* nz) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nx || !ny || !nz) return fail(GT_EINVAL, "gt_nonlinear: null field");
+  const KP kp = make_kp(c, false);
+  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh)) return rc;
+  double* dst[3] = {nx, ny, nz};
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, c->bh + f * c->fs, sizeof(cx<double>) * c->nxp,
+                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // bh must vanish outside the kept extents before the next step
+  if (int rc = set_mode(c, c->pruned)) return rc;
+  return GT_OK;
+}
+
+int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!ux || !uy || !uz) return fail(GT_EINVAL, "gt_velocity: null field");
+  void* w = nullptr;
+  double* phys = nullptr;
+  const size_t plane = (size_t)c->nx * c->ny * c->nz;
+  HIPCHK(hipMalloc(&w, sizeof(cx<double>) * 3 * c->fs));
+  if (hipMalloc((void**)&phys, sizeof(double) * 3 * plane) != hipSuccess) {
+    hipFree(w);
+    return fail(GT_ENOMEM, "gt_velocity: out of device memory");
+  }
+  int rc = GT_OK;
+  const KP kp = make_kp(c, false);
+  hipError_t e = launch_zinv(0, 3, kp, c->b, w, c->tw64[2], c->stream);
+  if (e == hipSuccess) e = launch_yfft(0, +1, 3, kp, w, c->tw64[1], c->stream);
+  if (e == hipSuccess) e = launch_xc2r(kp, w, phys, c->tw64[0], c->stream);
+  double* dst[3] = {ux, uy, uz};
+  for (int f = 0; f < 3 && e == hipSuccess; f++)
+    e = hipMemcpyAsync(dst[f], phys + f * plane, sizeof(double) * plane, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) rc = fail(GT_EHIP, std::string("gt_velocity: ") + hipGetErrorString(e));
+  hipFree(w);
+  hipFree(phys);
+  return rc;
+}
+
+int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double* uz) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!ux || !uy || !uz) return fail(GT_EINVAL, "gt_init_velocity: null field");
+  void* w = nullptr;
+  double* phys = nullptr;
+  const size_t plane = (size_t)c->nx * c->ny * c->nz;
+  HIPCHK(hipMalloc(&w, sizeof(cx<double>) * 3 * c->fs));
+  if (hipMalloc((void**)&phys, sizeof(double) * 3 * plane) != hipSuccess) {
+    hipFree(w);
+    return fail(GT_ENOMEM, "gt_init_velocity: out of device memory");
+  }
+  int rc = GT_OK;
+  const double* src[3] = {ux, uy, uz};
+  hipError_t e = hipSuccess;
+  for (int f = 0; f < 3 && e == hipSuccess; f++)
+    e = hipMemcpyAsync(phys + f * plane, src[f], sizeof(double) * plane, hipMemcpyHostToDevice, c->stream);
+  const KP kp = make_kp(c, true);
+  if (e == hipSuccess) e = hipMemsetAsync(w, 0, sizeof(cx<double>) * 3 * c->fs, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream);
+  if (e == hipSuccess) e = launch_xr2c(kp, phys, w, c->tw64[0], c->stream);
+  if (e == hipSuccess) e = launch_yfft(0, -1, 3, kp, w, c->tw64[1], c->stream);
+  if (e == hipSuccess) e = launch_zfwd(0, EPI_INIT, kp, w, nullptr, c->b, c->tw64[2], c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) rc = fail(GT_EHIP, std::string("gt_init_velocity: ") + hipGetErrorString(e));
+  hipFree(w);
+  hipFree(phys);
+  if (rc == GT_OK) rc = set_mode(c, c->p.truncate == 1);
+  if (rc == GT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GT_EHIP, "gt_init_velocity: sync");
+  return rc;
+}
+
+int gt_forcing_table(gt_ctx* c, int* nf, int* ikxf, int* ikyf, int* ikzf, double* ff, double* inpe, double* inpz,
+                     int cap) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nf) return fail(GT_EINVAL, "gt_forcing_table: null nf");
+  *nf = (int)c->forced.size();
+  if (inpe) *inpe = c->inpe;
+  if (inpz) *inpz = c->inpz;
+  const int n = std::min(cap, *nf);
+  for (int q = 0; q < n; q++) {
+    if (ikxf) ikxf[q] = c->forced[q].i;
+    if (ikyf) ikyf[q] = c->forced[q].j;
+    if (ikzf) ikzf[q] = c->forced[q].k;
+    if (ff) ff[q] = c->forced[q].ff;
+  }
+  return GT_OK;
+}
+
+int gt_forcing_count(const gt_params* p, int* nf, double* inpe, double* inpz) {
+  if (!p || !nf) return fail(GT_EINVAL, "gt_forcing_count: null argument");
+  if (p->nx < 2 || p->ny < 2 || p->nz < 2 || (p->nx | p->ny | p->nz) & 1)
+    return fail(GT_EINVAL, "gt_forcing_count: grid sizes must be even");
+  gt_ctx c;
+  c.p = *p;
+  c.nx = p->nx;
+  c.ny = p->ny;
+  c.nz = p->nz;
+  c.nxh = p->nx / 2 + 1;
+  c.nxp = (c.nxh + 15) / 16 * 16;
+  build_wavenumbers(&c);
+  if (build_forcing(&c)) return fail(GT_ENFORC, "gt_forcing_count: NFORC too small !");
+  *nf = (int)c.forced.size();
+  if (inpe) *inpe = c.inpe;
+  if (inpz) *inpz = c.inpz;
+  return GT_OK;
+}
+
+int gt_sync(gt_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GT_OK;
+}
+
+int gt_stream(gt_ctx* c, void** stream) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!stream) return fail(GT_EINVAL, "gt_stream: null");
+  *stream = (void*)c->stream;
+  return GT_OK;
+}
+
+int gt_pruned(gt_ctx* c, int* pruned) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!pruned) return fail(GT_EINVAL, "gt_pruned: null");
+  *pruned = c->pruned ? 1 : 0;
+  return GT_OK;
+}
+
+int gt_prof_enable(gt_ctx* c, int on) {
+  if (int rc = check_ctx(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  for (int q = 0; q < NPASS; q++) {
+    c->prof_ms[q] = 0;
+    c->prof_n[q] = 0;
+  }
+  c->prof = on != 0;
+  return GT_OK;
+}
+
+int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int npass) {
+  if (int rc = check_ctx(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    c->prof_ms[r.pass] += t;
+    c->prof_n[r.pass] += 1;
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  // algorithmic bytes of one launch of each pass
+  const double cw = c->prec ? 8.0 : 16.0, cb = 16.0;
+  const bool pr = c->pruned;
+  const int kxc = pr ? c->kxcut : c->nxh - 1;
+  auto keepcount = [](int N, int K) -> double { return K < 0 ? 0.0 : (K >= N / 2 ? (double)N : 2.0 * K + 1.0); };
+  double colsA = 0, elemsB = 0, keepE = 0, colsY = 0;
+  for (int i = 0; i <= kxc; i++) {
+    const int Ky = pr ? c->kyext[i] : c->ny / 2;
+    colsY += keepcount(c->ny, Ky);
+    for (int j = 0; j < c->ny; j++) {
+      const int K = pr ? c->kzext[(size_t)j * c->nxp + i] : c->nz / 2;
+      if (K >= 0) colsA += 1;
+      keepE += keepcount(c->nz, K);
+    }
+  }
+  (void)elemsB;
+  const double full = (double)(kxc + 1) * c->ny * c->nz;
+  double by[NPASS] = {0};
+  by[PASS_A] = keepE * 3 * cb + colsA * c->nz * 6 * cw;
+  by[PASS_B] = colsY * c->nz * 6 * cw + full * 6 * cw;
+  by[PASS_C] = full * 6 * cw + full * 3 * cw;
+  by[PASS_D] = full * 3 * cw + colsY * c->nz * 3 * cw;
+  by[PASS_E1] = colsA * c->nz * 3 * cw + keepE * 3 * cb + keepE * 3 * cb;
+  by[PASS_E2] = by[PASS_E1];
+  by[PASS_F] = (double)entries_per_step(c) * (sizeof(FEnt) + 6 * cb);
+  for (int q = 0; q < npass && q < NPASS; q++) {
+    if (ms) ms[q] = c->prof_ms[q];
+    if (launches) launches[q] = c->prof_n[q];
+    if (bytes) bytes[q] = by[q];
+  }
+  return GT_OK;
+}
+
+}  // extern "C"

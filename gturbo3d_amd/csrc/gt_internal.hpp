@@ -1,0 +1,65 @@
+// gt_internal.hpp -- shared definitions of the HIP library (kernels <-> host).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+
+namespace gt {
+
+// Per-launch description of the grid, the wavenumber tables and the
+// truncation-driven extents, passed by value to every kernel.
+//
+// Storage: every spectral field is [NZ][NY][NXP] complex with the row pitch
+// NXP = NX/2+1 rounded up to a multiple of 16, so every 16-wide tile of kx
+// starts on a 256-byte boundary.  Pad columns kx > NX/2 are never read.
+//
+// Extents (pruned mode, when truncate = 1 and the state is truncated):
+//   kxcut         largest kx index that any unmasked mode has
+//   kyext[kx]     largest |ky| index with an unmasked mode in column kx
+//   kzext[ky][kx] largest |kz| index unmasked at (kx, ky); -1 = none
+// keep(n, N, K) <=> min(n, N - n) <= K.  In dense mode every keep() is true.
+struct KP {
+  int nx, ny, nz, nxh, nxp;
+  long long fs;  // complex elements per field = nz*ny*nxp
+  const double* fkx;
+  const double* fky;
+  const double* fkz;
+  const double* fkxn;  // fkx^2 / fkxsmax (same expression as trunc.f90:14-16)
+  const double* fkyn;
+  const double* fkzn;
+  double rtrunc2;
+  int truncate;
+  int pruned;
+  int kxcut;
+  const int16_t* kyext;
+  const int16_t* kzext;
+  double omega, dt, dt2, nu, mu, alpnu, alpmu;
+  double scale;  // 1/(NX*NY*NZ), the fft_dir normalisation
+  int visc_simple;  // alpnu == 1 and mu == 0: scra = nu*k2
+};
+
+__device__ __forceinline__ bool keep_idx(int n, int N, int K) {
+  const int a = (n <= (N >> 1)) ? n : N - n;
+  return a <= K;
+}
+
+// One forcing increment (added to b at one spectral position).
+struct FEnt {
+  long long off;  // element offset within a field
+  double d[6];    // (re, im) for bx, by, bz
+};
+
+// pass identifiers for timing / byte accounting
+enum PassId {
+  PASS_A = 0,  // z inverse + curl/k^2 (3 -> 6 fields)
+  PASS_B = 1,  // y inverse (6 fields)
+  PASS_C = 2,  // x C2R, u x w, x R2C (6 -> 3 fields)
+  PASS_D = 3,  // y forward (3 fields)
+  PASS_E1 = 4, // z forward + Coriolis/invcurl/trunc + step1
+  PASS_E2 = 5, // z forward + Coriolis/invcurl/trunc + step2
+  PASS_F = 6,  // forcing scatter
+  NPASS = 7
+};
+
+}  // namespace gt

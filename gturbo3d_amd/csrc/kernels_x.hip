@@ -1,0 +1,289 @@
+// kernels_x.hip -- x-axis (contiguous row) passes and the forcing scatter.
+//
+// Every x kernel works on a pair of rows (y0, y0+1) at one z and packs the
+// same field of the two rows into one complex sequence, z = row0 + i*row1,
+// so a real transform of length NX costs half a complex one:
+//   C2R: Z[n] = X0[n] + i X1[n] for n <= NX/2 and conj(X0[NX-n]) + i conj(X1[NX-n])
+//        above, with the imaginary parts of X[0] and X[NX/2] dropped (the
+//        C2R of fft_inv, src/fft_inv.f90:12); row0 = Re z, row1 = Im z.
+//   R2C: Y = FFT(row0 + i row1), X0[k] = (Y[k] + conj Y[NX-k])/2,
+//        X1[k] = (Y[k] - conj Y[NX-k])/(2i) (the D2Z of src/fft_dir.f90:12).
+//
+// Pass C (k_xfused): the six inverse transforms of u and w, the cross
+//   product u x w of src/produ.f90:9-28 in LDS, and the three forward
+//   transforms, so the physical-space fields never reach HBM.
+// k_xc2r / k_xr2c: plain transforms for the velocity output
+//   (src/main.f90:196-201) and the initial transform (src/main.f90:91-93).
+#include "gt_config.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+
+template <typename W>
+__device__ __forceinline__ cx<W> herm_pack(cx<W> X0, cx<W> X1, bool conj_half) {
+  if (conj_half) return {X0.x + X1.y, -X0.y + X1.x};
+  return {X0.x - X1.y, X0.y + X1.x};
+}
+
+// Load the packed C2R input of one field at element n of the sequence.
+template <typename W, int N>
+__device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n, int kc) {
+  const int mm = (n <= N / 2) ? n : N - n;
+  cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
+  if (mm <= kc) {
+    X0 = r0[mm];
+    X1 = r1[mm];
+  }
+  if (mm == 0 || mm == N / 2) {
+    X0.y = W(0);
+    X1.y = W(0);
+  }
+  return herm_pack(X0, X1, n > N / 2);
+}
+
+template <typename W, int N, int P>
+__global__ __launch_bounds__(3 * P) void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
+  using L = LayRows<N>;
+  using FI = Fft<W, N, P, 2, +1, L>;
+  using FF = Fft<W, N, P, 1, -1, L>;
+  constexpr int STR = L::STR;
+  __shared__ cx<W> lds[6 * STR];
+  const int tid = threadIdx.x, sq = tid / P, t = tid % P;
+  const int y0 = 2 * blockIdx.x, z = blockIdx.y;
+  const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
+  const long long r1 = r0 + kp.nxp;
+  const int kc = kp.kxcut;
+
+  // six inverse transforms: thread slice sq handles fields sq and sq+3
+  {
+    cx<W> in[2][FI::E0], out[2][FI::EL];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const cx<W>* src = work + (sq + 3 * g) * kp.fs;
+#pragma unroll
+      for (int m = 0; m < FI::M0; m++)
+#pragma unroll
+        for (int r = 0; r < FI::R0; r++) {
+          cx<W> v = {W(0), W(0)};
+          if (FI::in_ok(t, m)) v = c2r_in<W, N>(src + r0, src + r1, FI::in_pos(t, m, r), kc);
+          in[g][m * FI::R0 + r] = v;
+        }
+    }
+    L lay{sq, 3};
+    FI::run(lay, lds, tw, t, in, out);
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+      for (int m = 0; m < FI::ML; m++)
+        if (FI::out_ok(t, m))
+#pragma unroll
+          for (int r = 0; r < FI::RL; r++) lds[lay.off(g, FI::out_pos(t, m, r))] = out[g][m * FI::RL + r];
+    __syncthreads();
+  }
+  // u x w (src/produ.f90:14-16 with a = u, b = w), component sq, both rows
+  cx<W> fin[1][FF::E0], fout[1][FF::EL];
+#pragma unroll
+  for (int m = 0; m < FF::M0; m++)
+#pragma unroll
+    for (int r = 0; r < FF::R0; r++) {
+      cx<W> v = {W(0), W(0)};
+      if (FF::in_ok(t, m)) {
+        const int n = FF::in_pos(t, m, r);
+        const int o = n + (n >> 3);
+        const cx<W> ux = lds[0 * STR + o], uy = lds[1 * STR + o], uz = lds[2 * STR + o];
+        const cx<W> wx = lds[3 * STR + o], wy = lds[4 * STR + o], wz = lds[5 * STR + o];
+        if (sq == 0) v = {uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y};
+        else if (sq == 1) v = {uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y};
+        else v = {ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y};
+      }
+      fin[0][m * FF::R0 + r] = v;
+    }
+  L lay1{sq, 0};
+  FF::run(lay1, lds, tw, t, fin, fout);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < FF::ML; m++)
+    if (FF::out_ok(t, m))
+#pragma unroll
+      for (int r = 0; r < FF::RL; r++) lds[lay1.off(0, FF::out_pos(t, m, r))] = fout[0][m * FF::RL + r];
+  __syncthreads();
+  cx<W>* dst = work + sq * kp.fs;
+  for (int k = t; k <= kc; k += P) {
+    const int kk = (k == 0) ? 0 : N - k;
+    const cx<W> Y = lds[lay1.off(0, k)], Yc = lds[lay1.off(0, kk)];
+    dst[r0 + k] = {(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)};
+    dst[r1 + k] = {(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)};
+  }
+}
+
+// velocity output: 3 half spectra -> 3 physical fields [nz][ny][nx]
+template <int N, int P>
+__global__ __launch_bounds__(3 * P) void k_xc2r(KP kp, const cx<double>* __restrict__ work, double* __restrict__ phys,
+                                                const cx<double>* __restrict__ tw) {
+  using L = LayRows<N>;
+  using FI = Fft<double, N, P, 1, +1, L>;
+  __shared__ cx<double> lds[3 * L::STR];
+  const int tid = threadIdx.x, sq = tid / P, t = tid % P;
+  const int y0 = 2 * blockIdx.x, z = blockIdx.y;
+  const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
+  const long long r1 = r0 + kp.nxp;
+  const cx<double>* src = work + sq * kp.fs;
+  cx<double> in[1][FI::E0], out[1][FI::EL];
+#pragma unroll
+  for (int m = 0; m < FI::M0; m++)
+#pragma unroll
+    for (int r = 0; r < FI::R0; r++) {
+      cx<double> v = {0.0, 0.0};
+      if (FI::in_ok(t, m)) v = c2r_in<double, N>(src + r0, src + r1, FI::in_pos(t, m, r), kp.kxcut);
+      in[0][m * FI::R0 + r] = v;
+    }
+  L lay{sq, 0};
+  FI::run(lay, lds, tw, t, in, out);
+  const long long plane = (long long)kp.nx * kp.ny * kp.nz;
+  double* d0 = phys + sq * plane + ((long long)z * kp.ny + y0) * kp.nx;
+  double* d1 = d0 + kp.nx;
+#pragma unroll
+  for (int m = 0; m < FI::ML; m++)
+    if (FI::out_ok(t, m))
+#pragma unroll
+      for (int r = 0; r < FI::RL; r++) {
+        const int n = FI::out_pos(t, m, r);
+        d0[n] = out[0][m * FI::RL + r].x;
+        d1[n] = out[0][m * FI::RL + r].y;
+      }
+}
+
+// initial transform: 3 physical fields -> 3 half spectra (all kx <= NX/2)
+template <int N, int P>
+__global__ __launch_bounds__(3 * P) void k_xr2c(KP kp, const double* __restrict__ phys, cx<double>* __restrict__ work,
+                                                const cx<double>* __restrict__ tw) {
+  using L = LayRows<N>;
+  using FF = Fft<double, N, P, 1, -1, L>;
+  __shared__ cx<double> lds[3 * L::STR];
+  const int tid = threadIdx.x, sq = tid / P, t = tid % P;
+  const int y0 = 2 * blockIdx.x, z = blockIdx.y;
+  const long long plane = (long long)kp.nx * kp.ny * kp.nz;
+  const double* s0 = phys + sq * plane + ((long long)z * kp.ny + y0) * kp.nx;
+  const double* s1 = s0 + kp.nx;
+  cx<double> in[1][FF::E0], out[1][FF::EL];
+#pragma unroll
+  for (int m = 0; m < FF::M0; m++)
+#pragma unroll
+    for (int r = 0; r < FF::R0; r++) {
+      cx<double> v = {0.0, 0.0};
+      if (FF::in_ok(t, m)) {
+        const int n = FF::in_pos(t, m, r);
+        v = {s0[n], s1[n]};
+      }
+      in[0][m * FF::R0 + r] = v;
+    }
+  L lay{sq, 0};
+  FF::run(lay, lds, tw, t, in, out);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < FF::ML; m++)
+    if (FF::out_ok(t, m))
+#pragma unroll
+      for (int r = 0; r < FF::RL; r++) lds[lay.off(0, FF::out_pos(t, m, r))] = out[0][m * FF::RL + r];
+  __syncthreads();
+  const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
+  const long long r1 = r0 + kp.nxp;
+  cx<double>* dst = work + sq * kp.fs;
+  for (int k = t; k <= N / 2; k += P) {
+    const int kk = (k == 0) ? 0 : N - k;
+    const cx<double> Y = lds[lay.off(0, k)], Yc = lds[lay.off(0, kk)];
+    dst[r0 + k] = {(Y.x + Yc.x) * 0.5, (Y.y - Yc.y) * 0.5};
+    dst[r1 + k] = {(Y.y + Yc.y) * 0.5, (Yc.x - Y.x) * 0.5};
+  }
+}
+
+// forcing: b[f][off] += d[f] for every entry (positions are distinct)
+__global__ void k_forcing(cx<double>* b, long long fs, const FEnt* __restrict__ ent, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FEnt e = ent[i];
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    cx<double>* q = b + f * fs + e.off;
+    cx<double> v = *q;
+    v.x = v.x + e.d[2 * f];
+    v.y = v.y + e.d[2 * f + 1];
+    *q = v;
+  }
+}
+
+// forcing in entry order by one thread, for tables whose positions repeat
+__global__ void k_forcing_seq(cx<double>* b, long long fs, const FEnt* __restrict__ ent, int n) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = 0; i < n; i++) {
+    const FEnt e = ent[i];
+    for (int f = 0; f < 3; f++) {
+      cx<double>* q = b + f * fs + e.off;
+      cx<double> v = *q;
+      v.x = v.x + e.d[2 * f];
+      v.y = v.y + e.d[2 * f + 1];
+      *q = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename W, int N>
+static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t s) {
+  constexpr int P = row_P(N);
+  dim3 grid(kp.ny / 2, kp.nz);
+  k_xfused<W, N, P><<<grid, 3 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_xfused(int prec, const KP& kp, void* work, const void* tw, hipStream_t s) {
+  switch (kp.nx) {
+#define GT_CASE(N) \
+  case N:          \
+    return prec ? xfused_n<float, N>(kp, work, tw, s) : xfused_n<double, N>(kp, work, tw, s);
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_xc2r(const KP& kp, const void* work, double* phys, const void* tw, hipStream_t s) {
+  dim3 grid(kp.ny / 2, kp.nz);
+  switch (kp.nx) {
+#define GT_CASE(N)                                                                                             \
+  case N:                                                                                                      \
+    k_xc2r<N, row_P(N)><<<grid, 3 * row_P(N), 0, s>>>(kp, (const cx<double>*)work, phys, (const cx<double>*)tw); \
+    return hipGetLastError();
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_xr2c(const KP& kp, const double* phys, void* work, const void* tw, hipStream_t s) {
+  dim3 grid(kp.ny / 2, kp.nz);
+  switch (kp.nx) {
+#define GT_CASE(N)                                                                                       \
+  case N:                                                                                                \
+    k_xr2c<N, row_P(N)><<<grid, 3 * row_P(N), 0, s>>>(kp, phys, (cx<double>*)work, (const cx<double>*)tw); \
+    return hipGetLastError();
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_forcing(void* b, long long fs, const FEnt* ent, int n, bool sequential, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (sequential)
+    k_forcing_seq<<<1, 64, 0, s>>>((cx<double>*)b, fs, ent, n);
+  else
+    k_forcing<<<(n + 127) / 128, 128, 0, s>>>((cx<double>*)b, fs, ent, n);
+  return hipGetLastError();
+}
+
+}  // namespace gt

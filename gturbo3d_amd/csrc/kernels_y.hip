@@ -1,0 +1,81 @@
+// kernels_y.hip -- the y-axis pencil passes (the y stages of the 3D
+// transforms behind fft_inv / fft_dir, src/fft_inv.f90:12, src/fft_dir.f90:12).
+//
+// Pass B (SIGN = +1) and pass D (SIGN = -1) transform every field of the
+// work array in place, one block per (kx tile, z, field).  In pruned mode
+// pass B reads only the ky rows that pass A wrote (the columns alive under
+// the truncation) and pass D writes only those rows back.
+#include "gt_config.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+
+template <typename W, int N, int T, int P, int SIGN>
+__global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
+  using F = Fft<W, N, P, 1, SIGN, LayInterleaved<T>>;
+  constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
+  __shared__ cx<W> lds[N * T];
+  const int tid = threadIdx.x, p = tid % T, t = tid / T;
+  const int kx = blockIdx.x * T + p;
+  const int z = blockIdx.y;
+  const int f = blockIdx.z;
+  const bool on = kx <= kp.kxcut;
+  const int Ky = on ? (kp.pruned ? (int)kp.kyext[kx] : (N >> 1)) : -1;
+  cx<W>* fld = work + f * kp.fs;
+  const int base = z * kp.ny * kp.nxp + kx;
+  const int ys = kp.nxp;
+  LayInterleaved<T> lay{p};
+
+  cx<W> in[1][M0 * R0], out[1][ML * RL];
+#pragma unroll
+  for (int m = 0; m < M0; m++)
+#pragma unroll
+    for (int r = 0; r < R0; r++) {
+      const int n = F::in_pos(t, m, r);
+      bool ld = on && F::in_ok(t, m);
+      if constexpr (SIGN > 0) ld = ld && keep_idx(n, N, Ky);
+      cx<W> v = {W(0), W(0)};
+      if (ld) v = fld[base + n * ys];
+      in[0][m * R0 + r] = v;
+    }
+  F::run(lay, lds, tw, t, in, out);
+  if (!on) return;
+#pragma unroll
+  for (int m = 0; m < ML; m++) {
+    if (!F::out_ok(t, m)) continue;
+#pragma unroll
+    for (int r = 0; r < RL; r++) {
+      const int n = F::out_pos(t, m, r);
+      if constexpr (SIGN < 0) {
+        if (!keep_idx(n, N, Ky)) continue;
+      }
+      fld[base + n * ys] = out[0][m * RL + r];
+    }
+  }
+}
+
+template <typename W, int N>
+static hipError_t yfft_n(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
+  constexpr int T = pencil_T(N), P = pencil_P(N);
+  dim3 grid((kp.kxcut + T) / T, kp.nz, nfields);
+  if (sign > 0)
+    k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  else
+    k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_yfft(int prec, int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
+  switch (kp.ny) {
+#define GT_CASE(N)                                                         \
+  case N:                                                                  \
+    return prec ? yfft_n<float, N>(sign, nfields, kp, work, tw, s)         \
+                : yfft_n<double, N>(sign, nfields, kp, work, tw, s);
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gt

@@ -1,0 +1,321 @@
+// kernels_z.hip -- the z-axis pencil passes of the step.
+//
+// Pass A (k_zinv): z-inverse transform of u = curl b and w = k^2 b, with the
+//   curl (src/produ.f90:44-63) and the k^2 scale (src/nlt.f90:26-40) applied
+//   to b as it is loaded, so b is read once and the six spectral fields of
+//   src/nlt.f90:16-40 never exist in HBM.
+// Pass E (k_zfwd): z-forward transform of the three products, then in
+//   registers: the fft_dir normalisation (src/fft_dir.f90:17-25), the
+//   Coriolis term (src/nlt.f90:67-81), invcurl (src/produ.f90:79-101),
+//   trunc (src/trunc.f90:10-28) and the integrating-factor update of step1
+//   (src/rkstep.f90:30-48) or step2 (src/rkstep.f90:86-104), with
+//   emk = exp(-0.5*dt*(nu*k2**alpnu + mu*k2**alpmu)) (src/defk.f90:57-68)
+//   evaluated on the fly instead of read from a table.
+//
+// Both passes touch only the (kx, ky, kz) the truncation leaves alive in
+// pruned mode (see KP in gt_internal.hpp).
+#include "gt_config.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+
+enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
+
+// Pass A, curl part: u = curl b for the three components (NOUT = 3).
+template <typename W, int N, int T, int P>
+__global__ __launch_bounds__(T* P) void k_zinv_curl(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
+                                                    const cx<W>* __restrict__ tw) {
+  using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
+  constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
+  __shared__ cx<W> lds[N * T];
+  const int tid = threadIdx.x, p = tid % T, t = tid / T;
+  const int kx = blockIdx.x * T + p;
+  const int j = blockIdx.y;
+  int K = -1;
+  if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
+  if (!__syncthreads_or(K >= 0)) return;
+
+  const int col = j * kp.nxp + kx;
+  const int zs = kp.ny * kp.nxp;
+  const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
+  const double fy = kp.fky[j];
+
+  cx<double> bv[3][M0 * R0];
+#pragma unroll
+  for (int f = 0; f < 3; f++) {
+    const cx<double>* bf = b + f * kp.fs;
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+#pragma unroll
+      for (int r = 0; r < R0; r++) {
+        const int n = F::in_pos(t, m, r);
+        cx<double> v = {0.0, 0.0};
+        if (F::in_ok(t, m) && keep_idx(n, N, K)) v = bf[col + n * zs];
+        bv[f][m * R0 + r] = v;
+      }
+  }
+
+  LayInterleaved<T> lay{p};
+#pragma unroll 1
+  for (int q = 0; q < 3; q++) {
+    cx<W> in[1][M0 * R0], out[1][ML * RL];
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+#pragma unroll
+      for (int r = 0; r < R0; r++) {
+        const int n = F::in_pos(t, m, r);
+        const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
+        // (src/produ.f90:48-53): u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
+        const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
+        const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
+        const cx<double> A = (q == 0) ? bv[2][m * R0 + r] : (q == 1) ? bv[0][m * R0 + r] : bv[1][m * R0 + r];
+        const cx<double> B = (q == 0) ? bv[1][m * R0 + r] : (q == 1) ? bv[2][m * R0 + r] : bv[0][m * R0 + r];
+        const double re = -ca * A.y + cb * B.y;
+        const double im = ca * A.x - cb * B.x;
+        in[0][m * R0 + r] = {(W)re, (W)im};
+      }
+    F::run(lay, lds, tw, t, in, out);
+    if (K >= 0) {
+      cx<W>* dst = work + q * kp.fs;
+#pragma unroll
+      for (int m = 0; m < ML; m++)
+        if (F::out_ok(t, m))
+#pragma unroll
+          for (int r = 0; r < RL; r++) dst[col + F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
+    }
+  }
+}
+
+// Pass A, vorticity part: w_f = k^2 b_f (src/nlt.f90:26-40), one field per
+// block (blockIdx.z), written to work field 3 + f.
+template <typename W, int N, int T, int P>
+__global__ __launch_bounds__(T* P) void k_zinv_k2(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
+                                                  const cx<W>* __restrict__ tw) {
+  using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
+  constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
+  __shared__ cx<W> lds[N * T];
+  const int tid = threadIdx.x, p = tid % T, t = tid / T;
+  const int kx = blockIdx.x * T + p;
+  const int j = blockIdx.y;
+  const int f = blockIdx.z;
+  int K = -1;
+  if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
+  if (!__syncthreads_or(K >= 0)) return;
+
+  const int col = j * kp.nxp + kx;
+  const int zs = kp.ny * kp.nxp;
+  const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
+  const double fy = kp.fky[j];
+  const double fxy = fx * fx + fy * fy;
+  const cx<double>* bf = b + f * kp.fs;
+  cx<W> in[1][M0 * R0], out[1][ML * RL];
+#pragma unroll
+  for (int m = 0; m < M0; m++)
+#pragma unroll
+    for (int r = 0; r < R0; r++) {
+      const int n = F::in_pos(t, m, r);
+      cx<double> v = {0.0, 0.0};
+      double fz = 0.0;
+      if (F::in_ok(t, m)) {
+        fz = kp.fkz[n];
+        if (keep_idx(n, N, K)) v = bf[col + n * zs];
+      }
+      const double k2 = fxy + fz * fz;  // fkxs + fkys + fkzs, in that order
+      in[0][m * R0 + r] = {(W)(k2 * v.x), (W)(k2 * v.y)};
+    }
+  LayInterleaved<T> lay{p};
+  F::run(lay, lds, tw, t, in, out);
+  if (K < 0) return;
+  cx<W>* dst = work + (3 + f) * kp.fs;
+#pragma unroll
+  for (int m = 0; m < ML; m++)
+    if (F::out_ok(t, m))
+#pragma unroll
+      for (int r = 0; r < RL; r++) dst[col + F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
+}
+
+__device__ __forceinline__ double emk_of(const KP& kp, double fk2) {
+  // src/defk.f90:61-65
+  double scra = 0.0;
+  if (fk2 != 0.0) {
+    if (kp.visc_simple) scra = kp.nu * fk2;
+    else scra = kp.nu * pow(fk2, kp.alpnu) + kp.mu * pow(fk2, kp.alpmu);
+  }
+  double e = exp(-0.5 * kp.dt * scra);
+  if (fk2 == 0.0) e = 1.0;
+  return e;
+}
+
+template <typename W, int N, int T, int P, int EPI>
+__global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
+                                               cx<double>* out, const cx<W>* __restrict__ tw) {
+  // out may be b itself (step2 updates b in place), so neither is restrict
+  using F = Fft<W, N, P, 1, -1, LayInterleaved<T>>;
+  constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
+  __shared__ cx<W> lds[N * T];
+  const int tid = threadIdx.x, p = tid % T, t = tid / T;
+  const int kx = blockIdx.x * T + p;
+  const int j = blockIdx.y;
+  int K = -1;
+  if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
+  if (!__syncthreads_or(K >= 0)) return;
+
+  const int col = j * kp.nxp + kx;
+  const int zs = kp.ny * kp.nxp;
+  LayInterleaved<T> lay{p};
+
+  cx<W> y[3][ML * RL];
+#pragma unroll 1
+  for (int f = 0; f < 3; f++) {
+    cx<W> in[1][M0 * R0], o[1][ML * RL];
+    const cx<W>* src = work + f * kp.fs + col;
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+#pragma unroll
+      for (int r = 0; r < R0; r++) {
+        cx<W> v = {W(0), W(0)};
+        if (K >= 0 && F::in_ok(t, m)) v = src[F::in_pos(t, m, r) * zs];
+        in[0][m * R0 + r] = v;
+      }
+    F::run(lay, lds, tw, t, in, o);
+#pragma unroll
+    for (int q = 0; q < ML * RL; q++) y[f][q] = o[0][q];
+  }
+  if (K < 0) return;
+
+  const int kxc = kx < kp.nxh ? kx : kp.nxh - 1;
+  const double fx = kp.fkx[kxc], fy = kp.fky[j];
+  const double fxs = fx * fx, fys = fy * fy;
+  const double kxn = kp.fkxn[kxc], kyn = kp.fkyn[j];
+#pragma unroll
+  for (int m = 0; m < ML; m++) {
+    if (!F::out_ok(t, m)) continue;
+#pragma unroll
+    for (int r = 0; r < RL; r++) {
+      const int n = F::out_pos(t, m, r);
+      if (!keep_idx(n, N, K)) continue;
+      const int q = col + n * zs;
+      const double fz = kp.fkz[n];
+      const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+      cx<double> a[3];
+#pragma unroll
+      for (int f = 0; f < 3; f++) a[f] = {(double)y[f][m * RL + r].x * kp.scale, (double)y[f][m * RL + r].y * kp.scale};
+      const double k2 = fxs + fys + fz * fz;
+      if constexpr (EPI == EPI_INIT) {
+        // incompr (src/trunc.f90:47-63), k = 0 skipped
+        if (!k0) {
+          double sr = fx * a[0].x + fy * a[1].x;
+          double si = fx * a[0].y + fy * a[1].y;
+          sr = sr + fz * a[2].x;
+          si = si + fz * a[2].y;
+          a[0] = {a[0].x - sr * fx / k2, a[0].y - si * fx / k2};
+          a[1] = {a[1].x - sr * fy / k2, a[1].y - si * fy / k2};
+          a[2] = {a[2].x - sr * fz / k2, a[2].y - si * fz / k2};
+        }
+      } else {
+        // Coriolis with the start-of-step b (src/nlt.f90:67-81)
+        const double of = kp.omega * fz;
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+          const cx<double> bb = b[f * kp.fs + q];
+          a[f] = {a[f].x - of * bb.y, a[f].y + of * bb.x};
+        }
+      }
+      // invcurl, k = 0 left untouched (src/produ.f90:83-99)
+      cx<double> nl[3] = {a[0], a[1], a[2]};
+      if (!k0) {
+        const double qx = fx / k2, qy = fy / k2, qz = fz / k2;
+        nl[0] = {-qy * a[2].y + qz * a[1].y, qy * a[2].x - qz * a[1].x};
+        nl[1] = {-qz * a[0].y + qx * a[2].y, qz * a[0].x - qx * a[2].x};
+        nl[2] = {-qx * a[1].y + qy * a[0].y, qx * a[1].x - qy * a[0].x};
+      }
+      if (kp.truncate == 1) {
+        // trunc (src/trunc.f90:14-19,27)
+        const double kn = (kp.fkzn[n] + kyn) + kxn;
+        if (kn > kp.rtrunc2 || k0) {
+#pragma unroll
+          for (int f = 0; f < 3; f++) nl[f] = {0.0, 0.0};
+        }
+      }
+      if constexpr (EPI == EPI_NONLIN || EPI == EPI_INIT) {
+#pragma unroll
+        for (int f = 0; f < 3; f++) out[f * kp.fs + q] = nl[f];
+      } else {
+        const double e = emk_of(kp, k2);
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+          const cx<double> bb = b[f * kp.fs + q];
+          cx<double> o;
+          if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
+          else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
+          if (k0) o = {0.0, 0.0};
+          out[f * kp.fs + q] = o;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename W, int N>
+static hipError_t zinv_n(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
+  constexpr int T = pencil_T(N), P = pencil_P(N);
+  dim3 grid((kp.kxcut + T) / T, kp.ny);
+  k_zinv_curl<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nout == 3) return e;
+  dim3 grid3((kp.kxcut + T) / T, kp.ny, 3);
+  k_zinv_k2<W, N, T, P><<<grid3, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
+  return hipGetLastError();
+}
+
+template <typename W, int N>
+static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
+                         hipStream_t s) {
+  constexpr int T = pencil_T(N), P = pencil_P(N);
+  dim3 grid((kp.kxcut + T) / T, kp.ny);
+  const cx<W>* w = (const cx<W>*)work;
+  const cx<double>* bb = (const cx<double>*)b;
+  cx<double>* o = (cx<double>*)out;
+  const cx<W>* t = (const cx<W>*)tw;
+  switch (epi) {
+    case EPI_STEP1: k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    case EPI_STEP2: k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_zinv(int prec, int nout, const KP& kp, const void* b, void* work, const void* tw,
+                       hipStream_t s) {
+  switch (kp.nz) {
+#define GT_CASE(N)                                                         \
+  case N:                                                                  \
+    return prec ? zinv_n<float, N>(nout, kp, b, work, tw, s)               \
+                : zinv_n<double, N>(nout, kp, b, work, tw, s);
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_zfwd(int prec, int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
+                       hipStream_t s) {
+  switch (kp.nz) {
+#define GT_CASE(N)                                                          \
+  case N:                                                                   \
+    return prec ? zfwd_n<float, N>(epi, kp, work, b, out, tw, s)            \
+                : zfwd_n<double, N>(epi, kp, work, b, out, tw, s);
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gt

@@ -1,0 +1,218 @@
+"""ctypes binding of libgturbo3d.so (include/gturbo3d.h).
+
+The library must have been built in-tree (``python -m gturbo3d_amd.build``
+or ``__graft_entry__.build()``).  There is no fallback: without the HIP
+library, or without a GPU, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Params
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgturbo3d.so")
+
+NPASS = 7
+PASS_NAMES = ["A_zinv_curl_k2", "B_yinv", "C_x_cross", "D_yfwd", "E1_zfwd_step1", "E2_zfwd_step2", "F_forcing"]
+
+# Every exported symbol of include/gturbo3d.h.
+EXPORTS = [
+    "gt_create", "gt_destroy", "gt_set_state", "gt_get_state", "gt_step", "gt_nonlinear",
+    "gt_velocity", "gt_init_velocity", "gt_forcing_table", "gt_forcing_count", "gt_sync", "gt_stream", "gt_pruned",
+    "gt_prof_enable", "gt_prof_read", "gt_last_error",
+]
+
+
+class GtParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xlx", ctypes.c_double), ("xly", ctypes.c_double), ("xlz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("alpnu", ctypes.c_double), ("mu", ctypes.c_double),
+        ("alpmu", ctypes.c_double), ("dt", ctypes.c_double), ("omega", ctypes.c_double),
+        ("rtrunc", ctypes.c_double), ("truncate", ctypes.c_int),
+        ("famp", ctypes.c_double), ("k1f", ctypes.c_double), ("k2f", ctypes.c_double),
+        ("variant", ctypes.c_int), ("dt2_mode", ctypes.c_int), ("precision", ctypes.c_int),
+    ]
+
+
+class GturboError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libgturbo3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GturboError(f"{LIB_PATH} not found: build it with `python -m gturbo3d_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip = ctypes.c_void_p, ctypes.c_int
+    L.gt_create.argtypes = [ctypes.POINTER(GtParams), ctypes.POINTER(vp)]
+    L.gt_destroy.argtypes = [vp]
+    L.gt_set_state.argtypes = [vp, vp, vp, vp]
+    L.gt_get_state.argtypes = [vp, vp, vp, vp]
+    L.gt_step.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int32)]
+    L.gt_nonlinear.argtypes = [vp, vp, vp, vp]
+    L.gt_velocity.argtypes = [vp, vp, vp, vp]
+    L.gt_init_velocity.argtypes = [vp, vp, vp, vp]
+    L.gt_forcing_table.argtypes = [vp, ctypes.POINTER(ip), vp, vp, vp, vp, vp, vp, ip]
+    L.gt_forcing_count.argtypes = [ctypes.POINTER(GtParams), ctypes.POINTER(ip), vp, vp]
+    L.gt_sync.argtypes = [vp]
+    L.gt_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.gt_pruned.argtypes = [vp, ctypes.POINTER(ip)]
+    L.gt_prof_enable.argtypes = [vp, ip]
+    L.gt_prof_read.argtypes = [vp, vp, vp, vp, ip]
+    L.gt_last_error.restype = ctypes.c_char_p
+    L.gt_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().gt_last_error().decode(errors="replace")
+        raise GturboError(f"{what} failed (code {rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def to_c(p: Params) -> GtParams:
+    p.validate()
+    return GtParams(p.nx, p.ny, p.nz, p.xlx, p.xly, p.xlz, p.nu, p.alpnu, p.mu, p.alpmu, p.dt,
+                    p.omega, p.rtrunc, p.truncate, p.famp, p.k1f, p.k2f, p.variant, p.dt2_mode,
+                    p.precision)
+
+
+def forcing_count(p: Params):
+    """(nf, inpe, inpz) of the forced-mode table for p, computed on the host."""
+    nf = ctypes.c_int(0)
+    inpe, inpz = ctypes.c_double(), ctypes.c_double()
+    cp = to_c(p)
+    _check(load().gt_forcing_count(ctypes.byref(cp), ctypes.byref(nf), ctypes.byref(inpe), ctypes.byref(inpz)),
+           "gt_forcing_count")
+    return nf.value, inpe.value, inpz.value
+
+
+class Context:
+    """One device context: the state b and everything the step needs.
+
+    Fields are numpy complex128 arrays of shape (NZ, NY, NX/2+1), i.e. the
+    reference's column-major complex*16 (NX2P1, NY, NZ) arrays.
+    """
+
+    def __init__(self, p: Params):
+        self.p = p
+        h = ctypes.c_void_p()
+        cp = to_c(p)
+        _check(load().gt_create(ctypes.byref(cp), ctypes.byref(h)), "gt_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load().gt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return self.p.shape
+
+    def _fields(self, fs, what):
+        out = []
+        for f in fs:
+            a = np.ascontiguousarray(f, dtype=np.complex128)
+            if a.shape != self.shape:
+                raise ValueError(f"{what}: expected shape {self.shape}, got {a.shape}")
+            out.append(a)
+        return out
+
+    def set_state(self, b) -> None:
+        b = self._fields(b, "set_state")
+        _check(load().gt_set_state(self.h, *[_ptr(x) for x in b]), "gt_set_state")
+
+    def get_state(self):
+        b = [np.empty(self.shape, np.complex128) for _ in range(3)]
+        _check(load().gt_get_state(self.h, *[_ptr(x) for x in b]), "gt_get_state")
+        return b
+
+    def step(self, nsteps: int, seed: int) -> int:
+        """Run nsteps steps; returns the advanced forcing seed."""
+        s = ctypes.c_int32(seed)
+        _check(load().gt_step(self.h, int(nsteps), ctypes.byref(s)), "gt_step")
+        return s.value
+
+    def sync(self) -> None:
+        _check(load().gt_sync(self.h), "gt_sync")
+
+    def nonlinear(self):
+        n = [np.empty(self.shape, np.complex128) for _ in range(3)]
+        _check(load().gt_nonlinear(self.h, *[_ptr(x) for x in n]), "gt_nonlinear")
+        return n
+
+    def velocity(self):
+        p = self.p
+        u = [np.empty((p.nz, p.ny, p.nx), np.float64) for _ in range(3)]
+        _check(load().gt_velocity(self.h, *[_ptr(x) for x in u]), "gt_velocity")
+        return u
+
+    def init_velocity(self, u) -> None:
+        p = self.p
+        us = [np.ascontiguousarray(f, dtype=np.float64) for f in u]
+        for f in us:
+            if f.shape != (p.nz, p.ny, p.nx):
+                raise ValueError(f"init_velocity: expected {(p.nz, p.ny, p.nx)}, got {f.shape}")
+        _check(load().gt_init_velocity(self.h, *[_ptr(x) for x in us]), "gt_init_velocity")
+
+    def forcing_table(self):
+        nf = ctypes.c_int(0)
+        _check(load().gt_forcing_table(self.h, ctypes.byref(nf), None, None, None, None, None, None, 0),
+               "gt_forcing_table")
+        n = nf.value
+        ikx, iky, ikz = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        ff = np.zeros(max(n, 1))
+        inpe, inpz = ctypes.c_double(), ctypes.c_double()
+        _check(load().gt_forcing_table(self.h, ctypes.byref(nf), _ptr(ikx), _ptr(iky), _ptr(ikz), _ptr(ff),
+                                       ctypes.byref(inpe), ctypes.byref(inpz), n), "gt_forcing_table")
+        return ikx[:n], iky[:n], ikz[:n], ff[:n], inpe.value, inpz.value
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(load().gt_stream(self.h, ctypes.byref(s)), "gt_stream")
+        return s.value or 0
+
+    def pruned(self) -> bool:
+        v = ctypes.c_int()
+        _check(load().gt_pruned(self.h, ctypes.byref(v)), "gt_pruned")
+        return bool(v.value)
+
+    def prof_enable(self, on: bool = True) -> None:
+        _check(load().gt_prof_enable(self.h, 1 if on else 0), "gt_prof_enable")
+
+    def prof_read(self):
+        ms = np.zeros(NPASS)
+        n = np.zeros(NPASS, np.int64)
+        by = np.zeros(NPASS)
+        _check(load().gt_prof_read(self.h, _ptr(ms), _ptr(n), _ptr(by), NPASS), "gt_prof_read")
+        return ms, n, by

@@ -1,0 +1,131 @@
+/*
+ * gturbo3d.h -- C ABI of the MI355X pseudo-spectral step library
+ * (libgturbo3d.so, built from gturbo3d_amd/csrc/).
+ *
+ * It replaces Gturbo3D's device step: the time-loop body of src/main.f90:124-150
+ * (bnl <- b; nlt; step1; nlt; step2; forcing) and the routines it calls,
+ * nlt (src/nlt.f90:1), step1/step2 (src/rkstep.f90:1,57), forcing
+ * (src/forcing_3d3c.f90:11, src/forcing_2d2c.f90:11), fft_dir/fft_inv
+ * (src/fft_dir.f90:1, src/fft_inv.f90:1) and the trunc/curl/invcurl/cross
+ * point operations.  The reference passes module globals and device arrays
+ * by implicit interface; here everything is an argument or lives in an
+ * opaque context, and every array crossing the ABI is a host array in the
+ * reference layout:
+ *
+ *   spectral field: complex128 (NX/2+1, NY, NZ) column-major = C order
+ *                   [NZ][NY][NX/2+1] of interleaved (re, im) doubles
+ *                   (src/main.f90:14);
+ *   physical field: real [NZ][NY][NX] doubles (the unpadded part of the
+ *                   in-place array of src/fft_inv.f90:9).
+ *
+ * Every function returns 0 on success and a nonzero GT_E* code otherwise;
+ * gt_last_error() returns a thread-local message for the last failure.  No
+ * C++ exception or abort crosses the ABI.  Calls on one context must be
+ * serialised by the caller; distinct contexts are independent.  The
+ * library owns all device memory; the caller owns host buffers and nothing
+ * is retained across calls.
+ */
+#ifndef GTURBO3D_H
+#define GTURBO3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_OK 0
+#define GT_EINVAL 1   /* bad argument */
+#define GT_EHIP 2     /* HIP runtime error */
+#define GT_ENOMEM 3   /* device or host allocation failed */
+#define GT_ENFORC 4   /* forced-mode table overflow ('NFORC too small !', src/iniforcing_3d3c.f90:33) */
+#define GT_ESIZE 5    /* grid size not supported by the compiled kernels */
+
+#define GT_FORCING_3D3C 3
+#define GT_FORCING_2D2C 2
+#define GT_DT2_REFERENCE 0 /* dt2 never assigned (src/modules.f90:32): bh = emk*b */
+#define GT_DT2_MIDPOINT 1  /* dt2 = dt/2 */
+#define GT_PREC_FP64 0
+#define GT_PREC_FP32_BULK 1
+
+/* Run parameters: params.dat (src/inieuler.f90:14-20) plus the grid
+ * (src/modules.f90:4), the forcing variant (Makefile:20-33), the stage-1
+ * coefficient mode and the precision. */
+typedef struct gt_params {
+  int nx, ny, nz;
+  double xlx, xly, xlz;
+  double nu, alpnu, mu, alpmu;
+  double dt, omega;
+  double rtrunc; /* unsquared, as in params.dat */
+  int truncate;
+  double famp, k1f, k2f;
+  int variant;
+  int dt2_mode;
+  int precision;
+} gt_params;
+
+typedef struct gt_ctx gt_ctx;
+
+/* Context with the tables of defk (src/defk.f90:1-94) and iniforcing
+ * (src/iniforcing_3d3c.f90:4-195 / src/iniforcing_2d2c.f90:4-88), the state
+ * b and its work arrays (src/main.f90:14, :71-85). */
+int gt_create(const gt_params* p, gt_ctx** out);
+int gt_destroy(gt_ctx* c);
+
+/* Copy the state b (3 spectral fields) in / out (src/main.f90:85, :202). */
+int gt_set_state(gt_ctx* c, const double* bx, const double* by, const double* bz);
+int gt_get_state(gt_ctx* c, double* bx, double* by, double* bz);
+
+/* nsteps iterations of the step body of src/main.f90:124-150.  *seed is the
+ * forcing LCG state (src/main.f90:42-45, src/forcing_3d3c.f90:82-93) on
+ * entry and is advanced on return.  Returns after enqueueing; results are
+ * visible to gt_get_state / gt_sync. */
+int gt_step(gt_ctx* c, int nsteps, int32_t* seed);
+
+/* N(b; b) = nlt(b, u, b) (src/nlt.f90:1-95) of the current state. */
+int gt_nonlinear(gt_ctx* c, double* nx, double* ny, double* nz);
+
+/* Physical velocity u = fft_inv(curl b) (src/main.f90:190-201). */
+int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz);
+
+/* Set b from a physical velocity: fft_dir x3, incompr, invcurl, trunc
+ * (src/main.f90:91-106). */
+int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double* uz);
+
+/* The forced-mode table (1-based indices as in the reference) and the mean
+ * energy / enstrophy input inpe, inpz (src/iniforcing_3d3c.f90:37-39). */
+int gt_forcing_table(gt_ctx* c, int* nf, int* ikxf, int* ikyf, int* ikzf, double* ff, double* inpe,
+                     double* inpz, int cap);
+
+/* Host only (no device work): the number of forced modes nf and the mean
+ * energy / enstrophy input that gt_create would build for *p (used to pick
+ * famp for a given injection rate, inpe = nf*famp^2). */
+int gt_forcing_count(const gt_params* p, int* nf, double* inpe, double* inpz);
+
+/* Wait for all work on the context's stream. */
+int gt_sync(gt_ctx* c);
+
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work after the step. */
+int gt_stream(gt_ctx* c, void** stream);
+
+/* 1 if the state is known to vanish outside the truncation ellipsoid and
+ * the step moves only the alive modes (pruned), 0 for the dense path. */
+int gt_pruned(gt_ctx* c, int* pruned);
+
+/* Per-pass timing with HIP events on the context stream.  Pass ids:
+ * 0 A (z inverse + curl, k^2), 1 B (y inverse), 2 C (x C2R, u x w, x R2C),
+ * 3 D (y forward), 4 E1 (z forward + step1), 5 E2 (z forward + step2),
+ * 6 F (forcing).  gt_prof_read waits for the stream and returns the summed
+ * milliseconds and launch counts since enable; bytes[] is the algorithmic
+ * HBM traffic of one launch of each pass (every element the pass must
+ * read or write, once). */
+int gt_prof_enable(gt_ctx* c, int on);
+int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int npass);
+
+const char* gt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTURBO3D_H */

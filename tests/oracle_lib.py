@@ -1,0 +1,255 @@
+"""ctypes wrapper of the CPU oracle (oracle/gt_oracle.c).
+
+Test infrastructure: only tests/, the smoke check and bench.py's CPU
+baseline use it, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class _OrParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xlx", ctypes.c_double), ("xly", ctypes.c_double), ("xlz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("alpnu", ctypes.c_double), ("mu", ctypes.c_double),
+        ("alpmu", ctypes.c_double), ("dt", ctypes.c_double), ("omega", ctypes.c_double),
+        ("rtrunc", ctypes.c_double), ("truncate", ctypes.c_int),
+        ("famp", ctypes.c_double), ("k1f", ctypes.c_double), ("k2f", ctypes.c_double),
+        ("variant", ctypes.c_int), ("dt2_mode", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build_oracle()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+        L.or_create.restype = vp
+        L.or_create.argtypes = [ctypes.POINTER(_OrParams)]
+        L.or_destroy.argtypes = [vp]
+        L.or_error.argtypes = [vp]
+        L.or_nf.argtypes = [vp]
+        L.or_rann.restype = ctypes.c_float
+        L.or_rann.argtypes = [ip]
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        for name in ("or_fft_dir", "or_fft_inv", "or_trunc"):
+            getattr(L, name).argtypes = [vp, vp]
+        for name in ("or_curl", "or_invcurl", "or_incompr", "or_init_from_velocity"):
+            getattr(L, name).argtypes = [vp, vp, vp, vp]
+        L.or_cross.argtypes = [vp] + [vp] * 6
+        L.or_nlt.argtypes = [vp] + [vp] * 9
+        L.or_step1.argtypes = [vp] + [vp] * 6
+        L.or_step2.argtypes = [vp] + [vp] * 6
+        L.or_forcing.argtypes = [vp, vp, vp, vp, ip]
+        L.or_step.argtypes = [vp] + [vp] * 9 + [ip, ctypes.c_int]
+        L.or_velocity.argtypes = [vp] + [vp] * 6
+        L.or_spectrum.argtypes = [vp] + [vp] * 5
+        L.or_fluxes.argtypes = [vp] + [vp] * 7
+        L.or_glob.argtypes = [vp, vp, vp, vp, vp]
+        L.or_forcing_table.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.or_tables.argtypes = [vp, vp, vp]
+        L.or_mask.argtypes = [vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def rann_sequence(seed: int, n: int):
+    """n successive rann draws from ``seed``; returns (values, final seed)."""
+    s = ctypes.c_int32(seed)
+    vals = [lib().or_rann(ctypes.byref(s)) for _ in range(n)]
+    return np.array(vals, dtype=np.float32), s.value
+
+
+class Oracle:
+    """One oracle context for a Params (gturbo3d_amd.params.Params)."""
+
+    def __init__(self, p):
+        self.p = p
+        s = _OrParams(p.nx, p.ny, p.nz, p.xlx, p.xly, p.xlz, p.nu, p.alpnu, p.mu, p.alpmu, p.dt,
+                      p.omega, p.rtrunc, p.truncate, p.famp, p.k1f, p.k2f, p.variant, p.dt2_mode)
+        self.h = lib().or_create(ctypes.byref(s))
+        if not self.h:
+            raise MemoryError("or_create failed")
+        err = lib().or_error(self.h)
+        if err:
+            raise RuntimeError(f"oracle init error {err} (2 = NFORC too small)")
+        self.shape = (p.nz, p.ny, p.nx // 2 + 1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    # ---- tables ----
+    @property
+    def nf(self) -> int:
+        return lib().or_nf(self.h)
+
+    def forcing_table(self):
+        nf = self.nf
+        ikx, iky, ikz = (np.zeros(max(nf, 1), np.int32) for _ in range(3))
+        ff = np.zeros(max(nf, 1))
+        inp = np.zeros(2)
+        lib().or_forcing_table(self.h, _p(ikx), _p(iky), _p(ikz), _p(ff), _p(inp))
+        return ikx[:nf], iky[:nf], ikz[:nf], ff[:nf], inp[0], inp[1]
+
+    def tables(self):
+        p = self.p
+        nxh = p.nx // 2 + 1
+        out = np.zeros(nxh + p.ny + p.nz + 4)
+        emk = np.zeros(self.shape)
+        lib().or_tables(self.h, _p(out), _p(emk))
+        fkx, fky, fkz = out[:nxh], out[nxh:nxh + p.ny], out[nxh + p.ny:nxh + p.ny + p.nz]
+        smax = out[-4:-1]
+        return dict(fkx=fkx, fky=fky, fkz=fkz, fkxsmax=smax[0], fkysmax=smax[1], fkzsmax=smax[2],
+                    rtrunc2=out[-1], emk=emk)
+
+    def mask(self):
+        m = np.zeros(self.shape, np.uint8)
+        lib().or_mask(self.h, _p(m))
+        return m.astype(bool)
+
+    # ---- transforms (padded real <-> spectral, in place on a complex array) ----
+    def fft_dir(self, phys):
+        """phys: real [nz][ny][nx] -> spectral complex [nz][ny][nxh] (fft_dir)."""
+        z = np.zeros(self.shape, np.complex128)
+        zr = z.view(np.float64).reshape(self.p.nz, self.p.ny, self.p.nx + 2)
+        zr[:, :, :self.p.nx] = phys
+        lib().or_fft_dir(self.h, _p(z))
+        return z
+
+    def fft_inv(self, spec):
+        z = np.ascontiguousarray(spec, dtype=np.complex128).copy()
+        lib().or_fft_inv(self.h, _p(z))
+        return z.view(np.float64).reshape(self.p.nz, self.p.ny, self.p.nx + 2)[:, :, :self.p.nx].copy()
+
+    # ---- point ops (return new arrays) ----
+    def _three(self, fn, a):
+        x = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
+        getattr(lib(), fn)(self.h, _p(x[0]), _p(x[1]), _p(x[2]))
+        return x
+
+    def curl(self, a):
+        return self._three("or_curl", a)
+
+    def invcurl(self, a):
+        return self._three("or_invcurl", a)
+
+    def incompr(self, a):
+        return self._three("or_incompr", a)
+
+    def trunc(self, f):
+        x = np.ascontiguousarray(f, dtype=np.complex128).copy()
+        lib().or_trunc(self.h, _p(x))
+        return x
+
+    def cross(self, a, b):
+        """a, b: physical real fields [nz][ny][nx]; returns a x b."""
+        p = self.p
+
+        def pad(f):
+            z = np.zeros((p.nz, p.ny, p.nx + 2))
+            z[:, :, :p.nx] = f
+            return z
+        A = [pad(f) for f in a]
+        B = [pad(f) for f in b]
+        lib().or_cross(self.h, *[_p(x) for x in A + B])
+        return [x[:, :, :p.nx].copy() for x in A]
+
+    def nlt(self, a, b):
+        """Returns (N, v) with N = nlt(a; b) and v = F[u x w]."""
+        A = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
+        V = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
+        lib().or_nlt(self.h, *[_p(x) for x in A + V + Bf])
+        return A, V
+
+    def step1(self, a, c):
+        C = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in c]
+        Af = [np.ascontiguousarray(f, dtype=np.complex128) for f in a]
+        lib().step1 if False else None
+        lib().or_step1(self.h, *[_p(x) for x in Af + C])
+        return C
+
+    def step2(self, a, c):
+        A = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
+        Cf = [np.ascontiguousarray(f, dtype=np.complex128) for f in c]
+        lib().or_step2(self.h, *[_p(x) for x in A + Cf])
+        return A
+
+    def forcing(self, b, seed: int):
+        B = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in b]
+        s = ctypes.c_int32(seed)
+        lib().or_forcing(self.h, _p(B[0]), _p(B[1]), _p(B[2]), ctypes.byref(s))
+        return B, s.value
+
+    def step(self, b, seed: int, nsteps: int = 1):
+        B = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in b]
+        W = [np.zeros(self.shape, np.complex128) for _ in range(6)]
+        s = ctypes.c_int32(seed)
+        lib().or_step(self.h, *[_p(x) for x in B + W], ctypes.byref(s), nsteps)
+        return B, s.value
+
+    def init_from_velocity(self, u):
+        p = self.p
+        Z = []
+        for f in u:
+            z = np.zeros(self.shape, np.complex128)
+            zr = z.view(np.float64).reshape(p.nz, p.ny, p.nx + 2)
+            zr[:, :, :p.nx] = f
+            Z.append(z)
+        lib().or_init_from_velocity(self.h, _p(Z[0]), _p(Z[1]), _p(Z[2]))
+        return Z
+
+    def velocity(self, b):
+        p = self.p
+        Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
+        U = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        lib().or_velocity(self.h, *[_p(x) for x in Bf + U])
+        return [u.view(np.float64).reshape(p.nz, p.ny, p.nx + 2)[:, :, :p.nx].copy() for u in U]
+
+    # ---- diagnostics ----
+    def spectrum(self, b):
+        nb = self.p.nx // 2
+        sp, sp2 = np.zeros(nb), np.zeros(nb)
+        Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
+        lib().or_spectrum(self.h, *[_p(x) for x in Bf], _p(sp), _p(sp2))
+        return sp, sp2
+
+    def fluxes(self, b, a):
+        nb = self.p.nx // 2
+        spf = np.zeros(nb)
+        X = [np.ascontiguousarray(f, dtype=np.complex128) for f in list(b) + list(a)]
+        lib().or_fluxes(self.h, *[_p(x) for x in X], _p(spf))
+        return spf
+
+    def glob(self, b):
+        out = np.zeros(18)
+        Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
+        lib().or_glob(self.h, *[_p(x) for x in Bf], _p(out))
+        return out
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(n)
