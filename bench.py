@@ -160,6 +160,8 @@ def main():
         "alg_bytes_per_step": alg_bytes_step,
         "alg_hbm_frac": alg_bytes_step * args.steps / t / (HBM_PEAK_GBS * 1e9),
         "pass_ms_per_step": {lib.PASS_NAMES[q]: ms[q] / args.steps for q in range(lib.NPASS)},
+        "pass_alg_gbs": {lib.PASS_NAMES[q]: (bytes_per[q] / (ms[q] / launches[q] * 1e-3) / 1e9 if launches[q] else 0.0)
+                         for q in range(lib.NPASS)},
         "state_finite": bool(finite),
         "roofline": roofline,
         "cpu_baseline": cpu,
