@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32-bulk"], default="fp64")
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", choices=["step", "copy"], default="step",
+                    help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -70,6 +73,17 @@ def main():
     import numpy as np
     import torch
     from gturbo3d_amd import initial, lib
+
+    if args.mode == "copy":
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                          "launches": 5}))
+        return
 
     n = args.size
     p0 = workload_params(n, args.precision)

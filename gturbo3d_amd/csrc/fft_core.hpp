@@ -193,10 +193,37 @@ struct LayRows {
   __device__ __forceinline__ int off(int g, int n) const { return (s0 + g * sstep) * STR + n + (n >> 3); }
 };
 
+// Rows one after another, element n of a row at n ^ ((n >> 4) & 15): the XOR
+// permutes each aligned block of 16 elements, so 16 consecutive lanes
+// reading consecutive n, or 8 lanes writing n = 8j + r, hit distinct
+// 16-byte bank groups, with no pad words.  For lengths with a partial last
+// block of 8 (24, 48) the permutation stays inside that block.
+template <int N>
+struct LaySwz {
+  int s0;     // first row of the thread
+  int sstep;  // row step between the thread's G rows
+  __device__ __forceinline__ int off(int g, int n) const { return (s0 + g * sstep) * N + (n ^ ((n >> 4) & 15)); }
+};
+
+// Synchronisation between the stages of a transform: the whole block, or
+// only the wavefront when every pencil of the transform lives in one wave.
+enum { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+
+template <int SYNC>
+__device__ __forceinline__ void stage_sync() {
+  if constexpr (SYNC == SYNC_BLOCK) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The staged transform.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int P, int G, int SIGN, class L>
+template <typename T, int N, int P, int G, int SIGN, class L, int SYNC = SYNC_BLOCK>
 struct Fft {
   using PL = Plan<N>;
   static constexpr int S = PL::S;
@@ -269,7 +296,7 @@ struct Fft {
 #pragma unroll
         for (int q = 0; q < M * R; q++) out[g][q] = v[g][q];
     } else {
-      __syncthreads();  // every read of the previous stage (or previous use) is done
+      stage_sync<SYNC>();  // every read of the previous stage (or previous use) is done
 #pragma unroll
       for (int m = 0; m < M; m++) {
         const int j = t + P * m;
@@ -282,7 +309,7 @@ struct Fft {
             for (int r = 0; r < R; r++) lds[lay.off(g, d + r * Ns)] = v[g][m * R + r];
         }
       }
-      __syncthreads();
+      stage_sync<SYNC>();
       stage<s + 1>(lay, lds, tw, t, in, out);
     }
   }

@@ -58,6 +58,7 @@ struct gt_ctx {
   int nx = 0, ny = 0, nz = 0, nxh = 0, nxp = 0;
   long long fs = 0;
   int prec = 0;
+  int resident_blocks = 1024;
   hipStream_t stream = nullptr;
 
   // host tables (src/defk.f90)
@@ -304,6 +305,7 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.alpmu = c->p.alpmu;
   kp.scale = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
   kp.visc_simple = (c->p.alpnu == 1.0 && c->p.mu == 0.0) ? 1 : 0;
+  kp.resident_blocks = c->resident_blocks;
   return kp;
 }
 
@@ -533,6 +535,12 @@ int gt_create(const gt_params* p, gt_ctx** out) {
                                                   std::string("gt_create: ") + what + ": " + hipGetErrorString(e));
   };
   chk(hipStreamCreate(&c->stream), "hipStreamCreate");
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      c->resident_blocks = 4 * ncu;
+  }
   const size_t cw = c->prec ? sizeof(cx<float>) : sizeof(cx<double>);
   if (rc == GT_OK) chk(hipMalloc(&c->b, sizeof(cx<double>) * 3 * c->fs), "state");
   if (rc == GT_OK) chk(hipMalloc(&c->bh, sizeof(cx<double>) * 3 * c->fs), "half-step state");
@@ -827,7 +835,7 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
   (void)elemsB;
   const double full = (double)(kxc + 1) * c->ny * c->nz;
   double by[NPASS] = {0};
-  by[PASS_A] = keepE * 3 * cb + colsA * c->nz * 6 * cw;
+  by[PASS_A] = keepE * 9 * cb + colsA * c->nz * 6 * cw;  // each of the 6 outputs reads 2 (curl) or 1 (k^2) fields of b
   by[PASS_B] = colsY * c->nz * 6 * cw + full * 6 * cw;
   by[PASS_C] = full * 6 * cw + full * 3 * cw;
   by[PASS_D] = full * 3 * cw + colsY * c->nz * 3 * cw;

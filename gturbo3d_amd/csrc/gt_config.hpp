@@ -29,7 +29,13 @@ __host__ __device__ constexpr int pencil_P(int n) {
   }
 }
 
-__host__ __device__ constexpr int pencil_T(int n) { return n >= 384 ? 8 : 16; }
+#ifndef GT_PENCIL_T_LARGE
+#define GT_PENCIL_T_LARGE 8
+#endif
+#ifndef GT_PENCIL_T_SMALL
+#define GT_PENCIL_T_SMALL 16
+#endif
+__host__ __device__ constexpr int pencil_T(int n) { return n >= 384 ? GT_PENCIL_T_LARGE : GT_PENCIL_T_SMALL; }
 
 // x passes: threads per row sequence (three sequences per block)
 __host__ __device__ constexpr int row_P(int n) { return pencil_P(n); }

@@ -37,6 +37,7 @@ struct KP {
   double omega, dt, dt2, nu, mu, alpnu, alpmu;
   double scale;  // 1/(NX*NY*NZ), the fft_dir normalisation
   int visc_simple;  // alpnu == 1 and mu == 0: scra = nu*k2
+  int resident_blocks;  // grid size of the persistent kernels (a few blocks per CU)
 };
 
 __device__ __forceinline__ bool keep_idx(int n, int N, int K) {

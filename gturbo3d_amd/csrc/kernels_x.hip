@@ -41,79 +41,97 @@ __device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n,
   return herm_pack(X0, X1, n > N / 2);
 }
 
+// One block per row pair, six groups of P threads (one wavefront for
+// N <= 512).  Group w inverse-transforms field w in its own LDS row; with one
+// wave per row the stage exchanges need only wavefront synchronisation.
+// The products are formed in registers at the forward transform's input
+// positions, groups 0-2 run the three forward transforms, and all six
+// groups unpack and store.
+#ifndef GT_XFUSED_WAVES_PER_EU
+#define GT_XFUSED_WAVES_PER_EU 1
+#endif
 template <typename W, int N, int P>
-__global__ __launch_bounds__(3 * P) void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
-  using L = LayRows<N>;
-  using FI = Fft<W, N, P, 2, +1, L>;
-  using FF = Fft<W, N, P, 1, -1, L>;
-  constexpr int STR = L::STR;
-  __shared__ cx<W> lds[6 * STR];
-  const int tid = threadIdx.x, sq = tid / P, t = tid % P;
+__global__ __launch_bounds__(6 * P) __attribute__((amdgpu_waves_per_eu(GT_XFUSED_WAVES_PER_EU)))
+void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
+  using L = LaySwz<N>;
+  constexpr int SY = (P <= 64) ? SYNC_WAVE : SYNC_BLOCK;
+  using FI = Fft<W, N, P, 1, +1, L, SY>;
+  using FF = Fft<W, N, P, 1, -1, L, SY>;
+  __shared__ cx<W> lds[6 * N];
+  const int tid = threadIdx.x, w = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
-  const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
-  const long long r1 = r0 + kp.nxp;
+  const int r0 = (z * kp.ny + y0) * kp.nxp;
+  const int r1 = r0 + kp.nxp;
   const int kc = kp.kxcut;
+  const L lay{w, 0};
 
-  // six inverse transforms: thread slice sq handles fields sq and sq+3
+  // inverse transform of field w (u_x, u_y, u_z, w_x, w_y, w_z), rows y0 and y0+1
   {
-    cx<W> in[2][FI::E0], out[2][FI::EL];
+    cx<W> in[1][FI::E0], out[1][FI::EL];
+    const cx<W>* src = work + w * kp.fs;
 #pragma unroll
-    for (int g = 0; g < 2; g++) {
-      const cx<W>* src = work + (sq + 3 * g) * kp.fs;
+    for (int m = 0; m < FI::M0; m++)
 #pragma unroll
-      for (int m = 0; m < FI::M0; m++)
-#pragma unroll
-        for (int r = 0; r < FI::R0; r++) {
-          cx<W> v = {W(0), W(0)};
-          if (FI::in_ok(t, m)) v = c2r_in<W, N>(src + r0, src + r1, FI::in_pos(t, m, r), kc);
-          in[g][m * FI::R0 + r] = v;
-        }
-    }
-    L lay{sq, 3};
+      for (int r = 0; r < FI::R0; r++) {
+        cx<W> v = {W(0), W(0)};
+        if (FI::in_ok(t, m)) v = c2r_in<W, N>(src + r0, src + r1, FI::in_pos(t, m, r), kc);
+        in[0][m * FI::R0 + r] = v;
+      }
     FI::run(lay, lds, tw, t, in, out);
-    __syncthreads();
+    stage_sync<SY>();
 #pragma unroll
-    for (int g = 0; g < 2; g++)
+    for (int m = 0; m < FI::ML; m++)
+      if (FI::out_ok(t, m))
 #pragma unroll
-      for (int m = 0; m < FI::ML; m++)
-        if (FI::out_ok(t, m))
-#pragma unroll
-          for (int r = 0; r < FI::RL; r++) lds[lay.off(g, FI::out_pos(t, m, r))] = out[g][m * FI::RL + r];
-    __syncthreads();
+        for (int r = 0; r < FI::RL; r++) lds[lay.off(0, FI::out_pos(t, m, r))] = out[0][m * FI::RL + r];
   }
-  // u x w (src/produ.f90:14-16 with a = u, b = w), component sq, both rows
+  __syncthreads();
+  // u x w (src/produ.f90:14-16 with a = u, b = w), component w, both rows
   cx<W> fin[1][FF::E0], fout[1][FF::EL];
+  const int comp = (w < 3) ? w : w - 3;
 #pragma unroll
   for (int m = 0; m < FF::M0; m++)
 #pragma unroll
     for (int r = 0; r < FF::R0; r++) {
       cx<W> v = {W(0), W(0)};
-      if (FF::in_ok(t, m)) {
+      if (w < 3 && FF::in_ok(t, m)) {
         const int n = FF::in_pos(t, m, r);
-        const int o = n + (n >> 3);
-        const cx<W> ux = lds[0 * STR + o], uy = lds[1 * STR + o], uz = lds[2 * STR + o];
-        const cx<W> wx = lds[3 * STR + o], wy = lds[4 * STR + o], wz = lds[5 * STR + o];
-        if (sq == 0) v = {uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y};
-        else if (sq == 1) v = {uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y};
-        else v = {ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y};
+        const int o = n ^ ((n >> 4) & 15);
+        if (comp == 0) {
+          const cx<W> uy = lds[1 * N + o], uz = lds[2 * N + o], wy = lds[4 * N + o], wz = lds[5 * N + o];
+          v = {uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y};
+        } else if (comp == 1) {
+          const cx<W> ux = lds[0 * N + o], uz = lds[2 * N + o], wx = lds[3 * N + o], wz = lds[5 * N + o];
+          v = {uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y};
+        } else {
+          const cx<W> ux = lds[0 * N + o], uy = lds[1 * N + o], wx = lds[3 * N + o], wy = lds[4 * N + o];
+          v = {ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y};
+        }
       }
       fin[0][m * FF::R0 + r] = v;
     }
-  L lay1{sq, 0};
-  FF::run(lay1, lds, tw, t, fin, fout);
-  __syncthreads();
+  __syncthreads();  // every product read is done before any forward transform writes LDS
+  if (SY == SYNC_BLOCK || w < 3) {
+    FF::run(lay, lds, tw, t, fin, fout);
+    stage_sync<SY>();
 #pragma unroll
-  for (int m = 0; m < FF::ML; m++)
-    if (FF::out_ok(t, m))
+    for (int m = 0; m < FF::ML; m++)
+      if (FF::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < FF::RL; r++) lds[lay1.off(0, FF::out_pos(t, m, r))] = fout[0][m * FF::RL + r];
+        for (int r = 0; r < FF::RL; r++) lds[lay.off(0, FF::out_pos(t, m, r))] = fout[0][m * FF::RL + r];
+  }
   __syncthreads();
-  cx<W>* dst = work + sq * kp.fs;
-  for (int k = t; k <= kc; k += P) {
-    const int kk = (k == 0) ? 0 : N - k;
-    const cx<W> Y = lds[lay1.off(0, k)], Yc = lds[lay1.off(0, kk)];
-    dst[r0 + k] = {(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)};
-    dst[r1 + k] = {(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)};
+  // unpack the two rows of component c = w % 3 (groups c and c+3 share it)
+  {
+    const int c = comp;
+    const L lc{c, 0};
+    cx<W>* dst = work + c * kp.fs;
+    for (int k = t + P * (w / 3); k <= kc; k += 2 * P) {
+      const int kk = (k == 0) ? 0 : N - k;
+      const cx<W> Y = lds[lc.off(0, k)], Yc = lds[lc.off(0, kk)];
+      dst[r0 + k] = {(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)};
+      dst[r1 + k] = {(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)};
+    }
   }
 }
 
@@ -121,9 +139,9 @@ __global__ __launch_bounds__(3 * P) void k_xfused(KP kp, cx<W>* work, const cx<W
 template <int N, int P>
 __global__ __launch_bounds__(3 * P) void k_xc2r(KP kp, const cx<double>* __restrict__ work, double* __restrict__ phys,
                                                 const cx<double>* __restrict__ tw) {
-  using L = LayRows<N>;
+  using L = LaySwz<N>;
   using FI = Fft<double, N, P, 1, +1, L>;
-  __shared__ cx<double> lds[3 * L::STR];
+  __shared__ cx<double> lds[3 * N];
   const int tid = threadIdx.x, sq = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
@@ -158,9 +176,9 @@ __global__ __launch_bounds__(3 * P) void k_xc2r(KP kp, const cx<double>* __restr
 template <int N, int P>
 __global__ __launch_bounds__(3 * P) void k_xr2c(KP kp, const double* __restrict__ phys, cx<double>* __restrict__ work,
                                                 const cx<double>* __restrict__ tw) {
-  using L = LayRows<N>;
+  using L = LaySwz<N>;
   using FF = Fft<double, N, P, 1, -1, L>;
-  __shared__ cx<double> lds[3 * L::STR];
+  __shared__ cx<double> lds[3 * N];
   const int tid = threadIdx.x, sq = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   const long long plane = (long long)kp.nx * kp.ny * kp.nz;
@@ -233,7 +251,7 @@ template <typename W, int N>
 static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = row_P(N);
   dim3 grid(kp.ny / 2, kp.nz);
-  k_xfused<W, N, P><<<grid, 3 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  k_xfused<W, N, P><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }
 

@@ -21,16 +21,21 @@ namespace gt {
 
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 
-// Pass A, curl part: u = curl b for the three components (NOUT = 3).
+// Pass A: one output component q = blockIdx.z per block (0..2: u = curl b,
+// 3..5: w = k^2 b), so a block holds only the one or two fields of b it
+// needs in registers.
+//   u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
+//   (src/produ.f90:48-53);  w_f = (fkxs + fkys + fkzs) b_f (src/nlt.f90:30-36)
 template <typename W, int N, int T, int P>
-__global__ __launch_bounds__(T* P) void k_zinv_curl(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
-                                                    const cx<W>* __restrict__ tw) {
+__global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
+                                               const cx<W>* __restrict__ tw) {
   using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int j = blockIdx.y;
+  const int q = blockIdx.z;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
   if (!__syncthreads_or(K >= 0)) return;
@@ -39,99 +44,50 @@ __global__ __launch_bounds__(T* P) void k_zinv_curl(KP kp, const cx<double>* __r
   const int zs = kp.ny * kp.nxp;
   const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
   const double fy = kp.fky[j];
+  const bool curl = q < 3;
+  // fields and wavenumber roles, uniform per block
+  const int fa = curl ? (q == 0 ? 2 : q == 1 ? 0 : 1) : q - 3;
+  const int fb = q == 0 ? 1 : q == 1 ? 2 : 0;
+  const cx<double>* A = b + fa * kp.fs + col;
+  const cx<double>* B = b + fb * kp.fs + col;
 
-  cx<double> bv[3][M0 * R0];
-#pragma unroll
-  for (int f = 0; f < 3; f++) {
-    const cx<double>* bf = b + f * kp.fs;
-#pragma unroll
-    for (int m = 0; m < M0; m++)
-#pragma unroll
-      for (int r = 0; r < R0; r++) {
-        const int n = F::in_pos(t, m, r);
-        cx<double> v = {0.0, 0.0};
-        if (F::in_ok(t, m) && keep_idx(n, N, K)) v = bf[col + n * zs];
-        bv[f][m * R0 + r] = v;
-      }
-  }
-
-  LayInterleaved<T> lay{p};
-#pragma unroll 1
-  for (int q = 0; q < 3; q++) {
-    cx<W> in[1][M0 * R0], out[1][ML * RL];
-#pragma unroll
-    for (int m = 0; m < M0; m++)
-#pragma unroll
-      for (int r = 0; r < R0; r++) {
-        const int n = F::in_pos(t, m, r);
-        const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
-        // (src/produ.f90:48-53): u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
-        const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
-        const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
-        const cx<double> A = (q == 0) ? bv[2][m * R0 + r] : (q == 1) ? bv[0][m * R0 + r] : bv[1][m * R0 + r];
-        const cx<double> B = (q == 0) ? bv[1][m * R0 + r] : (q == 1) ? bv[2][m * R0 + r] : bv[0][m * R0 + r];
-        const double re = -ca * A.y + cb * B.y;
-        const double im = ca * A.x - cb * B.x;
-        in[0][m * R0 + r] = {(W)re, (W)im};
-      }
-    F::run(lay, lds, tw, t, in, out);
-    if (K >= 0) {
-      cx<W>* dst = work + q * kp.fs;
-#pragma unroll
-      for (int m = 0; m < ML; m++)
-        if (F::out_ok(t, m))
-#pragma unroll
-          for (int r = 0; r < RL; r++) dst[col + F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
-    }
-  }
-}
-
-// Pass A, vorticity part: w_f = k^2 b_f (src/nlt.f90:26-40), one field per
-// block (blockIdx.z), written to work field 3 + f.
-template <typename W, int N, int T, int P>
-__global__ __launch_bounds__(T* P) void k_zinv_k2(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
-                                                  const cx<W>* __restrict__ tw) {
-  using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
-  constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<W> lds[N * T];
-  const int tid = threadIdx.x, p = tid % T, t = tid / T;
-  const int kx = blockIdx.x * T + p;
-  const int j = blockIdx.y;
-  const int f = blockIdx.z;
-  int K = -1;
-  if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
-  if (!__syncthreads_or(K >= 0)) return;
-
-  const int col = j * kp.nxp + kx;
-  const int zs = kp.ny * kp.nxp;
-  const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
-  const double fy = kp.fky[j];
-  const double fxy = fx * fx + fy * fy;
-  const cx<double>* bf = b + f * kp.fs;
   cx<W> in[1][M0 * R0], out[1][ML * RL];
 #pragma unroll
   for (int m = 0; m < M0; m++)
 #pragma unroll
     for (int r = 0; r < R0; r++) {
       const int n = F::in_pos(t, m, r);
-      cx<double> v = {0.0, 0.0};
+      cx<double> va = {0.0, 0.0}, vb = {0.0, 0.0};
       double fz = 0.0;
       if (F::in_ok(t, m)) {
         fz = kp.fkz[n];
-        if (keep_idx(n, N, K)) v = bf[col + n * zs];
+        if (keep_idx(n, N, K)) {
+          va = A[n * zs];
+          if (curl) vb = B[n * zs];
+        }
       }
-      const double k2 = fxy + fz * fz;  // fkxs + fkys + fkzs, in that order
-      in[0][m * R0 + r] = {(W)(k2 * v.x), (W)(k2 * v.y)};
+      double re, im;
+      if (curl) {
+        const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
+        const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
+        re = -ca * va.y + cb * vb.y;
+        im = ca * va.x - cb * vb.x;
+      } else {
+        const double k2 = fx * fx + fy * fy + fz * fz;
+        re = k2 * va.x;
+        im = k2 * va.y;
+      }
+      in[0][m * R0 + r] = {(W)re, (W)im};
     }
   LayInterleaved<T> lay{p};
   F::run(lay, lds, tw, t, in, out);
   if (K < 0) return;
-  cx<W>* dst = work + (3 + f) * kp.fs;
+  cx<W>* dst = work + q * kp.fs + col;
 #pragma unroll
   for (int m = 0; m < ML; m++)
     if (F::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < RL; r++) dst[col + F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
+      for (int r = 0; r < RL; r++) dst[F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
 }
 
 __device__ __forceinline__ double emk_of(const KP& kp, double fk2) {
@@ -165,7 +121,7 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
   LayInterleaved<T> lay{p};
 
   cx<W> y[3][ML * RL];
-#pragma unroll 1
+#pragma unroll
   for (int f = 0; f < 3; f++) {
     cx<W> in[1][M0 * R0], o[1][ML * RL];
     const cx<W>* src = work + f * kp.fs + col;
@@ -262,12 +218,8 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
 template <typename W, int N>
 static hipError_t zinv_n(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.ny);
-  k_zinv_curl<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || nout == 3) return e;
-  dim3 grid3((kp.kxcut + T) / T, kp.ny, 3);
-  k_zinv_k2<W, N, T, P><<<grid3, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
+  dim3 grid((kp.kxcut + T) / T, kp.ny, nout);
+  k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }
 

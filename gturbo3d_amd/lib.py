@@ -14,7 +14,7 @@ import numpy as np
 from .params import Params
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libgturbo3d.so")
+LIB_PATH = os.environ.get("GT_LIB_PATH") or os.path.join(PKG, "libgturbo3d.so")
 
 NPASS = 7
 PASS_NAMES = ["A_zinv_curl_k2", "B_yinv", "C_x_cross", "D_yfwd", "E1_zfwd_step1", "E2_zfwd_step2", "F_forcing"]
