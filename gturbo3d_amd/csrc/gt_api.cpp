@@ -24,6 +24,9 @@ hipError_t launch_xfused(int prec, const KP& kp, void* work, const void* tw, hip
 hipError_t launch_xc2r(const KP& kp, const void* work, double* phys, const void* tw, hipStream_t s);
 hipError_t launch_xr2c(const KP& kp, const double* phys, void* work, const void* tw, hipStream_t s);
 hipError_t launch_forcing(void* b, long long fs, const FEnt* ent, int n, bool sequential, hipStream_t s);
+hipError_t launch_diag(const KP& kp, const void* b, const void* nl, double* partial, int nblocks, double* out,
+                       int nbin, hipStream_t s);
+int diag_len(int nbin);
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 }  // namespace gt
 
@@ -89,6 +92,11 @@ struct gt_ctx {
   size_t hforce_cap = 0;
   hipEvent_t hforce_done = nullptr;
   bool hforce_pending = false;
+
+  // diagnostics scratch (allocated on first use)
+  double* ddiag_part = nullptr;
+  double* ddiag_out = nullptr;
+  int diag_blocks = 0;
 
   // profiling
   bool prof = false;
@@ -444,6 +452,8 @@ void free_ctx(gt_ctx* c) {
     hipFree(c->tw32[a]);
   }
   hipFree(c->dforce);
+  hipFree(c->ddiag_part);
+  hipFree(c->ddiag_out);
   if (c->hforce) hipHostFree(c->hforce);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -733,7 +743,61 @@ int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double
   return rc;
 }
 
-int gt_forcing_table(gt_ctx* c, int* nf, int* ikxf, int* ikyf, int* ikzf, double* ff, double* inpe, double* inpz,
+namespace {
+// one diagnostics sweep; host receives sp, sp2d, hel, flux bins and glob
+int run_diag(gt_ctx* c, const void* nl, std::vector<double>& host) {
+  const int nbin = c->nx / 2;
+  if (nbin > 512) return fail(GT_ESIZE, "diagnostics: NX/2 above 512 bins");
+  const int len = diag_len(nbin);
+  if (!c->ddiag_part) {
+    c->diag_blocks = std::max(64, c->resident_blocks);
+    HIPCHK(hipMalloc(&c->ddiag_part, sizeof(double) * (size_t)len * c->diag_blocks));
+    HIPCHK(hipMalloc(&c->ddiag_out, sizeof(double) * 4 * 512 + sizeof(double) * 18));
+  }
+  const KP kp = make_kp(c, false);
+  HIPCHK(launch_diag(kp, c->b, nl, c->ddiag_part, c->diag_blocks, c->ddiag_out, nbin, c->stream));
+  host.assign(len, 0.0);
+  HIPCHK(hipMemcpyAsync(host.data(), c->ddiag_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GT_OK;
+}
+}  // namespace
+
+extern "C" int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob) {
+  if (int rc = check_ctx(c)) return rc;
+  std::vector<double> h;
+  if (int rc = run_diag(c, nullptr, h)) return rc;
+  const int nbin = c->nx / 2;
+  for (int q = 0; q < nbin; q++) {
+    if (sp) sp[q] = h[q];
+    if (sp2d) sp2d[q] = h[nbin + q];
+    if (hel) hel[q] = h[2 * nbin + q];
+  }
+  if (glob)
+    for (int q = 0; q < 18; q++) glob[q] = h[4 * nbin + q];
+  return GT_OK;
+}
+
+extern "C" int gt_fluxes(gt_ctx* c, double* spf) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!spf) return fail(GT_EINVAL, "gt_fluxes: null output");
+  const KP kp = make_kp(c, false);
+  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh)) return rc;
+  std::vector<double> h;
+  int rc = run_diag(c, c->bh, h);
+  if (int r2 = set_mode(c, c->pruned)) return r2;
+  if (rc) return rc;
+  const int nbin = c->nx / 2;
+  const double* flux = h.data() + 3 * nbin;
+  for (int i = 1; i <= nbin; i++) {
+    double s = 0.0;
+    for (int q = i; q < nbin; q++) s += flux[q];
+    spf[i - 1] = s;
+  }
+  return GT_OK;
+}
+
+extern "C" int gt_forcing_table(gt_ctx* c, int* nf, int* ikxf, int* ikyf, int* ikzf, double* ff, double* inpe, double* inpz,
                      int cap) {
   if (int rc = check_ctx(c)) return rc;
   if (!nf) return fail(GT_EINVAL, "gt_forcing_table: null nf");

@@ -23,7 +23,7 @@ PASS_NAMES = ["A_zinv_curl_k2", "B_yinv", "C_x_cross", "D_yfwd", "E1_zfwd_step1"
 EXPORTS = [
     "gt_create", "gt_destroy", "gt_set_state", "gt_get_state", "gt_step", "gt_nonlinear",
     "gt_velocity", "gt_init_velocity", "gt_forcing_table", "gt_forcing_count", "gt_sync", "gt_stream", "gt_pruned",
-    "gt_prof_enable", "gt_prof_read", "gt_last_error",
+    "gt_prof_enable", "gt_prof_read", "gt_last_error", "gt_spectra", "gt_fluxes",
 ]
 
 
@@ -71,6 +71,8 @@ def load():
     L.gt_pruned.argtypes = [vp, ctypes.POINTER(ip)]
     L.gt_prof_enable.argtypes = [vp, ip]
     L.gt_prof_read.argtypes = [vp, vp, vp, vp, ip]
+    L.gt_spectra.argtypes = [vp, vp, vp, vp, vp]
+    L.gt_fluxes.argtypes = [vp, vp]
     L.gt_last_error.restype = ctypes.c_char_p
     L.gt_last_error.argtypes = []
     _lib = L
@@ -196,6 +198,18 @@ class Context:
         _check(load().gt_forcing_table(self.h, ctypes.byref(nf), _ptr(ikx), _ptr(iky), _ptr(ikz), _ptr(ff),
                                        ctypes.byref(inpe), ctypes.byref(inpz), n), "gt_forcing_table")
         return ikx[:n], iky[:n], ikz[:n], ff[:n], inpe.value, inpz.value
+
+    def spectra(self):
+        """(sp, sp2d, hel, glob) of the current state (see gt_spectra)."""
+        nb = self.p.nx // 2
+        sp, sp2d, hel, glob = np.zeros(nb), np.zeros(nb), np.zeros(nb), np.zeros(18)
+        _check(load().gt_spectra(self.h, _ptr(sp), _ptr(sp2d), _ptr(hel), _ptr(glob)), "gt_spectra")
+        return sp, sp2d, hel, glob
+
+    def fluxes(self):
+        spf = np.zeros(self.p.nx // 2)
+        _check(load().gt_fluxes(self.h, _ptr(spf)), "gt_fluxes")
+        return spf
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

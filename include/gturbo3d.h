@@ -92,6 +92,20 @@ int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz);
  * (src/main.f90:91-106). */
 int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double* uz);
 
+/* Diagnostics of the current state on the device (src/diagnostics.f90).
+ * sp, sp2d, hel: NX/2 doubles each; bin ib-1 holds the shell ib = int(|k|)+1
+ *   (the spectrum of src/diagnostics.f90:22-48: sum fac |k|^2 |b|^2 over the
+ *   shell, and the kz = 0 plane's |k|^2 |bz|^2; hel is sum fac Re(u*.w) with
+ *   u = curl b, w = curl u).  glob: 18 doubles, the nine sums of glob
+ *   (src/diagnostics.f90:126-153) for u followed by the nine for w
+ *   (:164-191).  Any output pointer may be NULL. */
+int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob);
+
+/* Energy flux spectrum of fluxes (src/diagnostics.f90:62-103): spf[i-1] =
+ * sum over shells q >= i of fac*|k|^2 Re(b.conj N), N = nlt(b,u,b), for
+ * i = 1..NX/2. */
+int gt_fluxes(gt_ctx* c, double* spf);
+
 /* The forced-mode table (1-based indices as in the reference) and the mean
  * energy / enstrophy input inpe, inpz (src/iniforcing_3d3c.f90:37-39). */
 int gt_forcing_table(gt_ctx* c, int* nf, int* ikxf, int* ikyf, int* ikzf, double* ff, double* inpe,

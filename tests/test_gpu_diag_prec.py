@@ -1,0 +1,105 @@
+"""Device diagnostics against the oracle, and the FP32-bulk precision variant
+against the FP64 oracle (error stated per field and per diagnostic)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from gturbo3d_amd import initial, lib
+from gturbo3d_amd.params import FORCING_2D2C, PREC_FP32_BULK, Params
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    num = math.sqrt(sum(float(np.sum(np.abs(x - y) ** 2)) for x, y in zip(a, b)))
+    den = math.sqrt(sum(float(np.sum(np.abs(y) ** 2)) for y in b))
+    return num / den
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def cfg(n, **kw):
+    base = dict(nx=n, ny=n, nz=n, nu=1e-3, dt=5e-3, omega=2.0, famp=0.05, k1f=2.0, k2f=3.0)
+    base.update(kw)
+    return Params(**base)
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_spectra_and_glob_match_oracle(require_gpu, n):
+    p = cfg(n)
+    say(f"spectra {n}^3")
+    b = initial.initial_state(p, seed=2)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        g.step(3, 11)
+        b3 = g.get_state()
+        sp, sp2d, hel, glob = g.spectra()
+    o = ol.Oracle(p)
+    osp, osp2d = o.spectrum(b3)
+    oglob = o.glob(b3)
+    assert np.allclose(sp, osp, rtol=1e-12, atol=1e-14 * osp.max())
+    assert np.allclose(sp2d, osp2d, rtol=1e-12, atol=1e-14 * max(osp2d.max(), 1e-300))
+    assert np.allclose(glob, oglob, rtol=1e-12, atol=1e-14 * np.abs(oglob).max())
+    # helicity: sum over shells = <u.w>/2 in physical space (w = curl u)
+    u = o.velocity(b3)
+    w = o.velocity(o.curl(b3))
+    h_phys = 0.5 * float(np.mean(u[0] * w[0] + u[1] * w[1] + u[2] * w[2]))
+    assert hel.sum() == pytest.approx(h_phys, rel=1e-10, abs=1e-14)
+    # energy in the spectrum equals glob's energy (solenoidal b)
+    assert sp.sum() == pytest.approx(glob[0] + glob[1] + glob[2], rel=1e-12)
+
+
+def test_fluxes_match_oracle(require_gpu):
+    p = cfg(32, omega=0.0)
+    say("fluxes 32^3")
+    b = initial.initial_state(p, seed=3)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        spf = g.fluxes()
+        b_after = g.get_state()
+    assert rel_l2(b_after, b) == 0.0  # fluxes leaves the state alone
+    o = ol.Oracle(p)
+    N, _ = o.nlt(b, b)
+    ospf = o.fluxes(b, N)
+    assert np.allclose(spf, ospf, rtol=0, atol=1e-11 * np.abs(ospf).max())
+    # the nonlinear term conserves energy: the flux through k = 0 vanishes
+    assert abs(spf[0]) < 1e-10 * np.abs(spf).max()
+
+
+@pytest.mark.parametrize("n,variant", [(32, 3), (64, 3), (64, FORCING_2D2C)])
+def test_fp32_bulk_against_fp64_oracle(require_gpu, n, variant):
+    kw = dict(precision=PREC_FP32_BULK, variant=variant)
+    if variant == FORCING_2D2C:
+        kw.update(k1f=3.0, k2f=4.0, famp=0.1, omega=5.0, nu=2e-4, dt=2e-3)
+    p = cfg(n, **kw)
+    say(f"fp32-bulk 10 steps {n}^3 variant {variant}")
+    b = initial.initial_state(p, seed=0)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        N = g.nonlinear()
+        s = g.step(10, 8299396)
+        b10 = g.get_state()
+        sp, _, hel, glob = g.spectra()
+    o = ol.Oracle(p)
+    No, _ = o.nlt(b, b)
+    ob10, os10 = o.step(b, 8299396, 10)
+    assert s == os10
+    e_nl = rel_l2(N, No)
+    e_fields = [rel_l2([x], [y]) for x, y in zip(b10, ob10)]
+    osp, _ = o.spectrum(ob10)
+    m = osp > 1e-12 * osp.max()
+    e_sp = float(np.max(np.abs(sp[m] - osp[m]) / osp[m]))
+    oglob = o.glob(ob10)
+    e_E = abs(sum(glob[:3]) - sum(oglob[:3])) / sum(oglob[:3])
+    say(f"  nlt rel L2 {e_nl:.2e}; fields after 10 steps {[f'{e:.2e}' for e in e_fields]}; "
+        f"spectrum max rel {e_sp:.2e}; energy rel {e_E:.2e}")
+    assert e_nl < 1e-5
+    assert max(e_fields) < 1e-5
+    assert e_sp < 1e-5
+    assert e_E < 1e-6
