@@ -62,6 +62,12 @@ struct gt_ctx {
   long long fs = 0;
   int prec = 0;
   int resident_blocks = 1024;
+  hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
+  // batches: nmem members with their own omega (and forcing seeds)
+  int nmem = 1;
+  std::vector<double> omegas;
+  double* domega = nullptr;
+  std::vector<char> mem_inside;  // member vanishes outside the kept extents
   hipStream_t stream = nullptr;
 
   // host tables (src/defk.f90)
@@ -314,6 +320,20 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.scale = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
   kp.visc_simple = (c->p.alpnu == 1.0 && c->p.mu == 0.0) ? 1 : 0;
   kp.resident_blocks = c->resident_blocks;
+  kp.nmem = c->nmem;
+  kp.bstride = 3 * c->fs;
+  kp.wstride = 6 * c->fs;
+  kp.omega_m = c->nmem > 1 ? c->domega : nullptr;
+  return kp;
+}
+
+// launch description of one member (velocity, init and diagnostics work on
+// one member at a time; the caller offsets the pointers)
+KP make_kp1(const gt_ctx* c, bool dense, int member) {
+  KP kp = make_kp(c, dense);
+  kp.nmem = 1;
+  kp.omega_m = nullptr;
+  if (c->nmem > 1) kp.omega = c->omegas[member];
   return kp;
 }
 
@@ -359,20 +379,20 @@ struct ProfScope {
   } while (0)
 
 // One evaluation of N(a; b) through passes A-D, then the pass-E epilogue.
-int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out) {
+int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out, hipStream_t st) {
   const int pr = c->prec;
-  LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), c->stream));
-  LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kp, c->work, twp(c, 1), c->stream));
-  LAUNCH(PASS_C, launch_xfused(pr, kp, c->work, twp(c, 0), c->stream));
-  LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), c->stream));
+  LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), st));
+  LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kp, c->work, twp(c, 1), st));
+  LAUNCH(PASS_C, launch_xfused(pr, kp, c->work, twp(c, 0), st));
+  LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), st));
   const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
-  LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), c->stream));
+  LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), st));
   return GT_OK;
 }
 
 // forcing increments of one step (src/forcing_3d3c.f90:26-72,
 // src/forcing_2d2c.f90:120-135); appends entries and advances the seed
-void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out) {
+void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out, int member = 0) {
   auto rann = [](int32_t* s) -> float {  // src/forcing_3d3c.f90:82-93
     const uint32_t mask = 0x7FFFFFFFu;
     uint32_t v = (69069u * (uint32_t)(*s) + 1u) & mask;
@@ -384,8 +404,14 @@ void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out) {
   size_t o = 0;
   for (const Forced& fm : c->forced) {
     const int i = fm.i, j = fm.j, k = fm.k;
-    const long long q = elem_off(c, i - 1, j - 1, k - 1);
-    const long long qc = elem_off(c, i - 1, c->jc[j - 1] - 1, c->kc[k - 1] - 1);
+    const long long mo = (long long)member * 3 * c->fs;
+    const long long q = mo + elem_off(c, i - 1, j - 1, k - 1);
+    const long long qc = mo + elem_off(c, i - 1, c->jc[j - 1] - 1, c->kc[k - 1] - 1);
+    if (!out) {  // draws only: advance the seed as the reference does
+      rann(seed);
+      if (c->p.variant != GT_FORCING_2D2C) rann(seed);
+      continue;
+    }
     if (c->p.variant == GT_FORCING_2D2C) {
       const double pi2 = 2.0 * 3.14159265358979;
       const double te = pi2 * (double)rann(seed);
@@ -443,6 +469,7 @@ void free_ctx(gt_ctx* c) {
   if (c->hforce_done) hipEventDestroy(c->hforce_done);
   hipFree(c->b);
   hipFree(c->bh);
+  hipFree(c->domega);
   hipFree(c->work);
   hipFree(c->dtab);
   hipFree(c->dkyext);
@@ -481,7 +508,7 @@ int ensure_force_cap(gt_ctx* c, size_t n) {
 // Set pruned mode and make bh vanish outside the kept extents.
 int set_mode(gt_ctx* c, bool pruned) {
   c->pruned = pruned;
-  HIPCHK(hipMemsetAsync(c->bh, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
+  HIPCHK(hipMemsetAsync(c->bh, 0, sizeof(cx<double>) * 3 * c->fs * c->nmem, c->stream));
   return GT_OK;
 }
 
@@ -496,8 +523,9 @@ extern "C" {
 
 const char* gt_last_error(void) { return g_err.c_str(); }
 
-int gt_create(const gt_params* p, gt_ctx** out) {
+static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx** out) {
   if (!p || !out) return fail(GT_EINVAL, "gt_create: null argument");
+  if (nmem < 1) return fail(GT_EINVAL, "gt_create: need at least one member");
   *out = nullptr;
   for (int n : {p->nx, p->ny, p->nz})
     if (!supported_n(n))
@@ -519,6 +547,11 @@ int gt_create(const gt_params* p, gt_ctx** out) {
   c->nxp = (c->nxh + 15) / 16 * 16;
   c->fs = (long long)c->nz * c->ny * c->nxp;
   c->prec = p->precision;
+  c->nmem = nmem;
+  c->omegas.assign(nmem, p->omega);
+  if (omegas)
+    for (int m = 0; m < nmem; m++) c->omegas[m] = omegas[m];
+  c->mem_inside.assign(nmem, 1);
   c->sdt = std::sqrt(p->dt);                              // src/inieuler.f90:67
   c->dt2 = (p->dt2_mode == GT_DT2_MIDPOINT) ? 0.5 * p->dt : 0.0;
   build_wavenumbers(c);
@@ -552,9 +585,13 @@ int gt_create(const gt_params* p, gt_ctx** out) {
       c->resident_blocks = 4 * ncu;
   }
   const size_t cw = c->prec ? sizeof(cx<float>) : sizeof(cx<double>);
-  if (rc == GT_OK) chk(hipMalloc(&c->b, sizeof(cx<double>) * 3 * c->fs), "state");
-  if (rc == GT_OK) chk(hipMalloc(&c->bh, sizeof(cx<double>) * 3 * c->fs), "half-step state");
-  if (rc == GT_OK) chk(hipMalloc(&c->work, cw * 6 * c->fs), "work fields");
+  const size_t nm = (size_t)nmem;
+  if (rc == GT_OK) chk(hipMalloc(&c->b, sizeof(cx<double>) * 3 * c->fs * nm), "state");
+  if (rc == GT_OK) chk(hipMalloc(&c->bh, sizeof(cx<double>) * 3 * c->fs * nm), "half-step state");
+  if (rc == GT_OK) chk(hipMalloc(&c->work, cw * 6 * c->fs * nm), "work fields");
+  if (rc == GT_OK) chk(hipMalloc(&c->domega, sizeof(double) * nm), "member omegas");
+  if (rc == GT_OK)
+    chk(hipMemcpy(c->domega, c->omegas.data(), sizeof(double) * nm, hipMemcpyHostToDevice), "member omegas");
   std::vector<double> tab;
   for (auto* v : {&c->fkx, &c->fky, &c->fkz, &c->fkxn, &c->fkyn, &c->fkzn}) tab.insert(tab.end(), v->begin(), v->end());
   if (rc == GT_OK) chk(hipMalloc(&c->dtab, sizeof(double) * tab.size()), "tables");
@@ -578,8 +615,8 @@ int gt_create(const gt_params* p, gt_ctx** out) {
     }
   }
   if (rc == GT_OK) chk(hipEventCreateWithFlags(&c->hforce_done, hipEventDisableTiming), "event");
-  if (rc == GT_OK) chk(hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream), "memset");
-  if (rc == GT_OK) chk(hipMemsetAsync(c->work, 0, cw * 6 * c->fs, c->stream), "memset");
+  if (rc == GT_OK) chk(hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs * nm, c->stream), "memset");
+  if (rc == GT_OK) chk(hipMemsetAsync(c->work, 0, cw * 6 * c->fs * nm, c->stream), "memset");
   if (rc == GT_OK) rc = set_mode(c, c->p.truncate == 1);
   if (rc == GT_OK) chk(hipStreamSynchronize(c->stream), "sync");
   if (rc != GT_OK) {
@@ -590,92 +627,130 @@ int gt_create(const gt_params* p, gt_ctx** out) {
   return GT_OK;
 }
 
+int gt_create(const gt_params* p, gt_ctx** out) { return create_ctx(p, 1, nullptr, out); }
+
 int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   free_ctx(c);
   return GT_OK;
 }
 
-int gt_set_state(gt_ctx* c, const double* bx, const double* by, const double* bz) {
+static int set_state_m(gt_ctx* c, int member, const double* bx, const double* by, const double* bz) {
   if (int rc = check_ctx(c)) return rc;
-  if (!bx || !by || !bz) return fail(GT_EINVAL, "gt_set_state: null field");
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  if (!bx || !by || !bz) return fail(GT_EINVAL, "set_state: null field");
+  if (member < 0 || member >= c->nmem) return fail(GT_EINVAL, "set_state: member out of range");
   const double* src[3] = {bx, by, bz};
-  // pruned only if every field vanishes outside the kept extents
+  // pruned only if every member vanishes outside the kept extents
   bool inside = (c->p.truncate == 1);
   for (int f = 0; f < 3 && inside; f++) {
-    const double* s = src[f];
-    for (int k = 0; k < c->nz && inside; k++)
-      for (int j = 0; j < c->ny && inside; j++) {
-        const int K = c->kzext[(size_t)j * c->nxp];  // column test below is per kx
-        (void)K;
-        const int mk = std::min(k, c->nz - k);
+    const double* sf = src[f];
+    for (int k = 0; k < c->nz && inside; k++) {
+      const int mk = std::min(k, c->nz - k);
+      for (int j = 0; j < c->ny && inside; j++)
         for (int i = 0; i < c->nxh; i++) {
-          const int Kc = c->kzext[(size_t)j * c->nxp + i];
-          if (mk <= Kc) continue;
+          if (mk <= c->kzext[(size_t)j * c->nxp + i]) continue;
           const size_t q = (((size_t)k * c->ny + j) * c->nxh + i) * 2;
-          if (s[q] != 0.0 || s[q + 1] != 0.0) {
+          if (sf[q] != 0.0 || sf[q + 1] != 0.0) {
             inside = false;
             break;
           }
         }
-      }
+    }
   }
-  HIPCHK(hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
+  cx<double>* dst = c->b + (size_t)member * 3 * c->fs;
+  HIPCHK(hipMemsetAsync(dst, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
   for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(c->b + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
+    HIPCHK(hipMemcpy2DAsync(dst + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
                             sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->stream));
-  if (int rc = set_mode(c, inside)) return rc;
+  c->mem_inside[member] = inside ? 1 : 0;
+  bool all = true;
+  for (char v : c->mem_inside) all = all && v;
+  if (int rc = set_mode(c, all)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return GT_OK;
 }
 
-int gt_get_state(gt_ctx* c, double* bx, double* by, double* bz) {
+static int get_state_m(gt_ctx* c, int member, double* bx, double* by, double* bz) {
   if (int rc = check_ctx(c)) return rc;
-  if (!bx || !by || !bz) return fail(GT_EINVAL, "gt_get_state: null field");
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  if (!bx || !by || !bz) return fail(GT_EINVAL, "get_state: null field");
+  if (member < 0 || member >= c->nmem) return fail(GT_EINVAL, "get_state: member out of range");
   double* dst[3] = {bx, by, bz};
+  const cx<double>* src = c->b + (size_t)member * 3 * c->fs;
   for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, c->b + f * c->fs, sizeof(cx<double>) * c->nxp,
+    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, src + f * c->fs, sizeof(cx<double>) * c->nxp,
                             sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GT_OK;
 }
 
-int gt_step(gt_ctx* c, int nsteps, int32_t* seed) {
-  if (int rc = check_ctx(c)) return rc;
-  if (!seed) return fail(GT_EINVAL, "gt_step: null seed");
-  if (nsteps < 0) return fail(GT_EINVAL, "gt_step: negative step count");
-  if (nsteps == 0) return GT_OK;
+int gt_set_state(gt_ctx* c, const double* bx, const double* by, const double* bz) {
+  return set_state_m(c, 0, bx, by, bz);
+}
+
+int gt_get_state(gt_ctx* c, double* bx, double* by, double* bz) { return get_state_m(c, 0, bx, by, bz); }
+
+// Enqueue nsteps steps of every member on stream st; the forcing entries of
+// step s are at dforce + s * per * nmem.
+static int enqueue_steps(gt_ctx* c, hipStream_t st, int nsteps, const FEnt* dforce) {
   const size_t per = entries_per_step(c);
-  if (per > 0) {
-    if (int rc = ensure_force_cap(c, per * (size_t)nsteps)) return rc;
-    if (c->hforce_pending) {
-      HIPCHK(hipEventSynchronize(c->hforce_done));
-      c->hforce_pending = false;
-    }
-    for (int s = 0; s < nsteps; s++) forcing_entries(c, seed, c->hforce + per * s);
-    HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * per * nsteps, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->hforce_done, c->stream));
-    c->hforce_pending = true;
-  } else {
-    for (int s = 0; s < nsteps; s++) forcing_entries(c, seed, nullptr);
-  }
   const KP kp = make_kp(c, false);
   for (int s = 0; s < nsteps; s++) {
     // stage 1: N1 = nlt(b; b), bh = emk (b + dt2 N1)      (src/main.f90:124-141)
-    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh)) return rc;
+    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, st)) return rc;
     // stage 2: N2 = nlt(bh; b), b = emk (emk b + dt N2)   (src/main.f90:144-146)
-    if (int rc = nonlinear_stage(c, kp, c->bh, EPI_STEP2, c->b)) return rc;
+    if (int rc = nonlinear_stage(c, kp, c->bh, EPI_STEP2, c->b, st)) return rc;
     // forcing (src/main.f90:149)
-    if (per > 0) LAUNCH(PASS_F, launch_forcing(c->b, c->fs, c->dforce + per * s, (int)per, c->force_seq, c->stream));
+    if (per > 0)
+      LAUNCH(PASS_F, launch_forcing(c->b, c->fs, dforce + per * c->nmem * s, (int)(per * c->nmem), c->force_seq, st));
   }
   return GT_OK;
 }
 
+// Host side of the forcing for nsteps steps of every member into buf
+// (laid out [step][member][entry]), advancing seeds[member].
+static void fill_forcing(const gt_ctx* c, int nsteps, int32_t* seeds, FEnt* buf) {
+  const size_t per = entries_per_step(c);
+  for (int s = 0; s < nsteps; s++)
+    for (int m = 0; m < c->nmem; m++) forcing_entries(c, &seeds[m], per ? buf + per * (c->nmem * s + m) : nullptr, m);
+}
+
+static int step_m(gt_ctx* c, int nsteps, int32_t* seeds) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  if (!seeds) return fail(GT_EINVAL, "step: null seed");
+  if (nsteps < 0) return fail(GT_EINVAL, "step: negative step count");
+  if (nsteps == 0) return GT_OK;
+  const size_t per = entries_per_step(c);
+  const size_t need = per * (size_t)c->nmem * (size_t)nsteps;
+  if (need > 0) {
+    if (int rc = ensure_force_cap(c, need)) return rc;
+    if (c->hforce_pending) {
+      HIPCHK(hipEventSynchronize(c->hforce_done));
+      c->hforce_pending = false;
+    }
+    fill_forcing(c, nsteps, seeds, c->hforce);
+    HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * need, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->hforce_done, c->stream));
+    c->hforce_pending = true;
+  } else {
+    fill_forcing(c, nsteps, seeds, nullptr);
+  }
+  return enqueue_steps(c, c->stream, nsteps, c->dforce);
+}
+
+int gt_step(gt_ctx* c, int nsteps, int32_t* seed) {
+  if (c && c->nmem != 1) return fail(GT_EINVAL, "gt_step: batch contexts step with gt_batch_step");
+  return step_m(c, nsteps, seed);
+}
+
 int gt_nonlinear(gt_ctx* c, double* nx, double* ny, double* nz) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, "gt_nonlinear: single-member contexts only");
   if (!nx || !ny || !nz) return fail(GT_EINVAL, "gt_nonlinear: null field");
   const KP kp = make_kp(c, false);
-  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh)) return rc;
+  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream)) return rc;
   double* dst[3] = {nx, ny, nz};
   for (int f = 0; f < 3; f++)
     HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, c->bh + f * c->fs, sizeof(cx<double>) * c->nxp,
@@ -688,6 +763,7 @@ int gt_nonlinear(gt_ctx* c, double* nx, double* ny, double* nz) {
 
 int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, "gt_velocity: single-member contexts only");
   if (!ux || !uy || !uz) return fail(GT_EINVAL, "gt_velocity: null field");
   void* w = nullptr;
   double* phys = nullptr;
@@ -698,7 +774,7 @@ int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz) {
     return fail(GT_ENOMEM, "gt_velocity: out of device memory");
   }
   int rc = GT_OK;
-  const KP kp = make_kp(c, false);
+  const KP kp = make_kp1(c, false, 0);
   hipError_t e = launch_zinv(0, 3, kp, c->b, w, c->tw64[2], c->stream);
   if (e == hipSuccess) e = launch_yfft(0, +1, 3, kp, w, c->tw64[1], c->stream);
   if (e == hipSuccess) e = launch_xc2r(kp, w, phys, c->tw64[0], c->stream);
@@ -714,6 +790,7 @@ int gt_velocity(gt_ctx* c, double* ux, double* uy, double* uz) {
 
 int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double* uz) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, "gt_init_velocity: single-member contexts only");
   if (!ux || !uy || !uz) return fail(GT_EINVAL, "gt_init_velocity: null field");
   void* w = nullptr;
   double* phys = nullptr;
@@ -728,7 +805,7 @@ int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double
   hipError_t e = hipSuccess;
   for (int f = 0; f < 3 && e == hipSuccess; f++)
     e = hipMemcpyAsync(phys + f * plane, src[f], sizeof(double) * plane, hipMemcpyHostToDevice, c->stream);
-  const KP kp = make_kp(c, true);
+  const KP kp = make_kp1(c, true, 0);
   if (e == hipSuccess) e = hipMemsetAsync(w, 0, sizeof(cx<double>) * 3 * c->fs, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->b, 0, sizeof(cx<double>) * 3 * c->fs, c->stream);
   if (e == hipSuccess) e = launch_xr2c(kp, phys, w, c->tw64[0], c->stream);
@@ -745,7 +822,7 @@ int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double
 
 namespace {
 // one diagnostics sweep; host receives sp, sp2d, hel, flux bins and glob
-int run_diag(gt_ctx* c, const void* nl, std::vector<double>& host) {
+int run_diag(gt_ctx* c, int member, const void* nl, std::vector<double>& host) {
   const int nbin = c->nx / 2;
   if (nbin > 512) return fail(GT_ESIZE, "diagnostics: NX/2 above 512 bins");
   const int len = diag_len(nbin);
@@ -754,8 +831,9 @@ int run_diag(gt_ctx* c, const void* nl, std::vector<double>& host) {
     HIPCHK(hipMalloc(&c->ddiag_part, sizeof(double) * (size_t)len * c->diag_blocks));
     HIPCHK(hipMalloc(&c->ddiag_out, sizeof(double) * 4 * 512 + sizeof(double) * 18));
   }
-  const KP kp = make_kp(c, false);
-  HIPCHK(launch_diag(kp, c->b, nl, c->ddiag_part, c->diag_blocks, c->ddiag_out, nbin, c->stream));
+  const KP kp = make_kp1(c, false, member);
+  HIPCHK(launch_diag(kp, c->b + (size_t)member * 3 * c->fs, nl, c->ddiag_part, c->diag_blocks, c->ddiag_out, nbin,
+                     c->stream));
   host.assign(len, 0.0);
   HIPCHK(hipMemcpyAsync(host.data(), c->ddiag_out, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -763,10 +841,12 @@ int run_diag(gt_ctx* c, const void* nl, std::vector<double>& host) {
 }
 }  // namespace
 
-extern "C" int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob) {
+static int spectra_m(gt_ctx* c, int member, double* sp, double* sp2d, double* hel, double* glob) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  if (member < 0 || member >= c->nmem) return fail(GT_EINVAL, "spectra: member out of range");
   std::vector<double> h;
-  if (int rc = run_diag(c, nullptr, h)) return rc;
+  if (int rc = run_diag(c, member, nullptr, h)) return rc;
   const int nbin = c->nx / 2;
   for (int q = 0; q < nbin; q++) {
     if (sp) sp[q] = h[q];
@@ -778,13 +858,18 @@ extern "C" int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, doub
   return GT_OK;
 }
 
+extern "C" int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob) {
+  return spectra_m(c, 0, sp, sp2d, hel, glob);
+}
+
 extern "C" int gt_fluxes(gt_ctx* c, double* spf) {
   if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, "gt_fluxes: single-member contexts only");
   if (!spf) return fail(GT_EINVAL, "gt_fluxes: null output");
   const KP kp = make_kp(c, false);
-  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh)) return rc;
+  if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream)) return rc;
   std::vector<double> h;
-  int rc = run_diag(c, c->bh, h);
+  int rc = run_diag(c, 0, c->bh, h);
   if (int r2 = set_mode(c, c->pruned)) return r2;
   if (rc) return rc;
   const int nbin = c->nx / 2;
@@ -911,6 +996,204 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
     if (launches) launches[q] = c->prof_n[q];
     if (bytes) bytes[q] = by[q];
   }
+  return GT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches and ensembles
+// ---------------------------------------------------------------------------
+
+int gt_batch_create(const gt_params* p, int nmembers, const double* omegas, gt_ctx** out) {
+  return create_ctx(p, nmembers, omegas, out);
+}
+
+int gt_batch_size(gt_ctx* c, int* nmembers) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nmembers) return fail(GT_EINVAL, "gt_batch_size: null");
+  *nmembers = c->nmem;
+  return GT_OK;
+}
+
+int gt_batch_set_state(gt_ctx* c, int member, const double* bx, const double* by, const double* bz) {
+  return set_state_m(c, member, bx, by, bz);
+}
+
+int gt_batch_get_state(gt_ctx* c, int member, double* bx, double* by, double* bz) {
+  return get_state_m(c, member, bx, by, bz);
+}
+
+int gt_batch_step(gt_ctx* c, int nsteps, int32_t* seeds) { return step_m(c, nsteps, seeds); }
+
+int gt_batch_spectra(gt_ctx* c, int member, double* sp, double* sp2d, double* hel, double* glob) {
+  return spectra_m(c, member, sp, sp2d, hel, glob);
+}
+
+}  // extern "C"
+
+struct gt_ensemble {
+  std::vector<gt_ctx*> batches;
+  std::vector<std::vector<int32_t>> seeds;  // per batch, per member
+  int chunk = 0;
+  hipStream_t stream = nullptr;
+  // two captured graphs, each reading its own pinned forcing buffers
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<FEnt*> hbuf[2];  // per batch
+  hipEvent_t done[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  int next = 0;
+};
+
+namespace {
+
+void free_ensemble(gt_ensemble* e) {
+  if (!e) return;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (gt_ctx* c : e->batches)
+    if (c && c->ext_stream == e->stream) c->ext_stream = nullptr;
+  for (int g = 0; g < 2; g++) {
+    if (e->exec[g]) hipGraphExecDestroy(e->exec[g]);
+    if (e->graph[g]) hipGraphDestroy(e->graph[g]);
+    for (FEnt* h : e->hbuf[g])
+      if (h) hipHostFree(h);
+    if (e->done[g]) hipEventDestroy(e->done[g]);
+  }
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int ensemble_capture(gt_ensemble* e, int g) {
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = GT_OK;
+  for (size_t q = 0; q < e->batches.size() && rc == GT_OK; q++) {
+    gt_ctx* c = e->batches[q];
+    const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)e->chunk;
+    if (need > 0 && hipMemcpyAsync(c->dforce, e->hbuf[g][q], sizeof(FEnt) * need, hipMemcpyHostToDevice, e->stream) !=
+                        hipSuccess)
+      rc = fail(GT_EHIP, "ensemble capture: forcing copy");
+    if (rc == GT_OK) rc = enqueue_steps(c, e->stream, e->chunk, c->dforce);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t err = hipStreamEndCapture(e->stream, &graph);
+  if (rc != GT_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (err != hipSuccess) return fail(GT_EHIP, std::string("ensemble capture: ") + hipGetErrorString(err));
+  e->graph[g] = graph;
+  HIPCHK(hipGraphInstantiate(&e->exec[g], graph, nullptr, nullptr, 0));
+  return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gt_ensemble_create(gt_ctx** batches, int nbatches, int chunk, gt_ensemble** out) {
+  if (!batches || nbatches < 1 || !out) return fail(GT_EINVAL, "gt_ensemble_create: bad arguments");
+  if (chunk < 1) return fail(GT_EINVAL, "gt_ensemble_create: chunk must be >= 1");
+  *out = nullptr;
+  gt_ensemble* e = new (std::nothrow) gt_ensemble();
+  if (!e) return fail(GT_ENOMEM, "gt_ensemble_create: out of host memory");
+  e->chunk = chunk;
+  int rc = GT_OK;
+  for (int q = 0; q < nbatches; q++) {
+    if (!batches[q]) rc = fail(GT_EINVAL, "gt_ensemble_create: null batch");
+    e->batches.push_back(batches[q]);
+    e->seeds.emplace_back(batches[q] ? batches[q]->nmem : 0, 0);
+  }
+  if (rc == GT_OK && hipStreamCreate(&e->stream) != hipSuccess) rc = fail(GT_EHIP, "gt_ensemble_create: stream");
+  for (int g = 0; g < 2 && rc == GT_OK; g++) {
+    if (hipEventCreateWithFlags(&e->done[g], hipEventDisableTiming) != hipSuccess)
+      rc = fail(GT_EHIP, "gt_ensemble_create: event");
+    for (gt_ctx* c : e->batches) {
+      if (rc != GT_OK) break;
+      c->prof = false;  // no timing events inside a captured graph
+      const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)chunk;
+      FEnt* h = nullptr;
+      if (need > 0) {
+        if (int r = ensure_force_cap(c, need)) rc = r;
+        else if (hipHostMalloc((void**)&h, sizeof(FEnt) * need, hipHostMallocDefault) != hipSuccess)
+          rc = fail(GT_ENOMEM, "gt_ensemble_create: pinned forcing buffer");
+      }
+      e->hbuf[g].push_back(h);
+    }
+  }
+  for (gt_ctx* c : e->batches) {
+    if (rc == GT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GT_EHIP, "gt_ensemble_create: sync");
+    if (rc == GT_OK) c->ext_stream = e->stream;
+  }
+  for (int g = 0; g < 2 && rc == GT_OK; g++) rc = ensemble_capture(e, g);
+  if (rc != GT_OK) {
+    free_ensemble(e);
+    return rc;
+  }
+  *out = e;
+  return GT_OK;
+}
+
+int gt_ensemble_destroy(gt_ensemble* e) {
+  free_ensemble(e);
+  return GT_OK;
+}
+
+int gt_ensemble_set_seeds(gt_ensemble* e, int batch, const int32_t* seeds) {
+  if (!e || batch < 0 || batch >= (int)e->batches.size() || !seeds)
+    return fail(GT_EINVAL, "gt_ensemble_set_seeds: bad arguments");
+  for (int m = 0; m < e->batches[batch]->nmem; m++) e->seeds[batch][m] = seeds[m];
+  return GT_OK;
+}
+
+int gt_ensemble_get_seeds(gt_ensemble* e, int batch, int32_t* seeds) {
+  if (!e || batch < 0 || batch >= (int)e->batches.size() || !seeds)
+    return fail(GT_EINVAL, "gt_ensemble_get_seeds: bad arguments");
+  for (int m = 0; m < e->batches[batch]->nmem; m++) seeds[m] = e->seeds[batch][m];
+  return GT_OK;
+}
+
+// Advance every member of every batch by nsteps: whole chunks replay the
+// captured graphs (alternating, so the host fills one pinned buffer while the
+// other graph runs); a remainder runs eagerly on the ensemble stream.
+int gt_ensemble_step(gt_ensemble* e, int nsteps) {
+  if (!e) return fail(GT_EINVAL, "gt_ensemble_step: null ensemble");
+  if (nsteps < 0) return fail(GT_EINVAL, "gt_ensemble_step: negative step count");
+  int left = nsteps;
+  while (left >= e->chunk) {
+    const int g = e->next;
+    if (e->used[g]) HIPCHK(hipEventSynchronize(e->done[g]));
+    for (size_t q = 0; q < e->batches.size(); q++)
+      fill_forcing(e->batches[q], e->chunk, e->seeds[q].data(), e->hbuf[g][q]);
+    HIPCHK(hipGraphLaunch(e->exec[g], e->stream));
+    HIPCHK(hipEventRecord(e->done[g], e->stream));
+    e->used[g] = true;
+    e->next ^= 1;
+    left -= e->chunk;
+  }
+  if (left > 0) {
+    HIPCHK(hipStreamSynchronize(e->stream));  // the device forcing buffers are shared with the graphs
+    for (size_t q = 0; q < e->batches.size(); q++) {
+      gt_ctx* c = e->batches[q];
+      const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)left;
+      if (need > 0) {
+        if (c->hforce_pending) {
+          HIPCHK(hipEventSynchronize(c->hforce_done));
+          c->hforce_pending = false;
+        }
+        fill_forcing(c, left, e->seeds[q].data(), c->hforce);
+        HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * need, hipMemcpyHostToDevice, e->stream));
+      } else {
+        fill_forcing(c, left, e->seeds[q].data(), nullptr);
+      }
+      if (int rc = enqueue_steps(c, e->stream, left, c->dforce)) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  return GT_OK;
+}
+
+int gt_ensemble_sync(gt_ensemble* e) {
+  if (!e) return fail(GT_EINVAL, "gt_ensemble_sync: null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
   return GT_OK;
 }
 

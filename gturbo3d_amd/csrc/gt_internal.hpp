@@ -38,6 +38,11 @@ struct KP {
   double scale;  // 1/(NX*NY*NZ), the fft_dir normalisation
   int visc_simple;  // alpnu == 1 and mu == 0: scra = nu*k2
   int resident_blocks;  // grid size of the persistent kernels (a few blocks per CU)
+  // batches: nmem independent members, member m's fields start m*bstride
+  // (state arrays) or m*wstride (work array) elements further on
+  int nmem;
+  long long bstride, wstride;
+  const double* omega_m;  // per-member omega, or nullptr for omega
 };
 
 __device__ __forceinline__ bool keep_idx(int n, int N, int K) {

@@ -60,6 +60,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   __shared__ cx<W> lds[6 * N];
   const int tid = threadIdx.x, w = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
+  work += blockIdx.z * kp.wstride;
   const int r0 = (z * kp.ny + y0) * kp.nxp;
   const int r1 = r0 + kp.nxp;
   const int kc = kp.kxcut;
@@ -250,7 +251,7 @@ __global__ void k_forcing_seq(cx<double>* b, long long fs, const FEnt* __restric
 template <typename W, int N>
 static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = row_P(N);
-  dim3 grid(kp.ny / 2, kp.nz);
+  dim3 grid(kp.ny / 2, kp.nz, kp.nmem);
   k_xfused<W, N, P><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }

@@ -11,14 +11,15 @@
 namespace gt {
 
 template <typename W, int N, int T, int P, int SIGN>
-__global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
+__global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* __restrict__ tw, int nfields) {
   using F = Fft<W, N, P, 1, SIGN, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int z = blockIdx.y;
-  const int f = blockIdx.z;
+  const int f = blockIdx.z % nfields;
+  work += (blockIdx.z / nfields) * kp.wstride;
   const bool on = kx <= kp.kxcut;
   const int Ky = on ? (kp.pruned ? (int)kp.kyext[kx] : (N >> 1)) : -1;
   cx<W>* fld = work + f * kp.fs;
@@ -57,11 +58,11 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
 template <typename W, int N>
 static hipError_t yfft_n(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.nz, nfields);
+  dim3 grid((kp.kxcut + T) / T, kp.nz, nfields * kp.nmem);
   if (sign > 0)
-    k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
   else
-    k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
   return hipGetLastError();
 }
 

@@ -28,14 +28,17 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 //   (src/produ.f90:48-53);  w_f = (fkxs + fkys + fkzs) b_f (src/nlt.f90:30-36)
 template <typename W, int N, int T, int P>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
-                                               const cx<W>* __restrict__ tw) {
+                                               const cx<W>* __restrict__ tw, int nout) {
   using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int j = blockIdx.y;
-  const int q = blockIdx.z;
+  const int q = blockIdx.z % nout;
+  const int mem = blockIdx.z / nout;
+  b += mem * kp.bstride;
+  work += mem * kp.wstride;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
   if (!__syncthreads_or(K >= 0)) return;
@@ -112,6 +115,11 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int j = blockIdx.y;
+  const int mem = blockIdx.z;
+  work += mem * kp.wstride;
+  if (b) b += mem * kp.bstride;
+  out += mem * kp.bstride;
+  const double omega = kp.omega_m ? kp.omega_m[mem] : kp.omega;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
   if (!__syncthreads_or(K >= 0)) return;
@@ -170,7 +178,7 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
         }
       } else {
         // Coriolis with the start-of-step b (src/nlt.f90:67-81)
-        const double of = kp.omega * fz;
+        const double of = omega * fz;
 #pragma unroll
         for (int f = 0; f < 3; f++) {
           const cx<double> bb = b[f * kp.fs + q];
@@ -218,8 +226,8 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
 template <typename W, int N>
 static hipError_t zinv_n(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.ny, nout);
-  k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw);
+  dim3 grid((kp.kxcut + T) / T, kp.ny, nout * kp.nmem);
+  k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
   return hipGetLastError();
 }
 
@@ -227,7 +235,7 @@ template <typename W, int N>
 static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
                          hipStream_t s) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.ny);
+  dim3 grid((kp.kxcut + T) / T, kp.ny, kp.nmem);
   const cx<W>* w = (const cx<W>*)work;
   const cx<double>* bb = (const cx<double>*)b;
   cx<double>* o = (cx<double>*)out;

@@ -24,6 +24,9 @@ EXPORTS = [
     "gt_create", "gt_destroy", "gt_set_state", "gt_get_state", "gt_step", "gt_nonlinear",
     "gt_velocity", "gt_init_velocity", "gt_forcing_table", "gt_forcing_count", "gt_sync", "gt_stream", "gt_pruned",
     "gt_prof_enable", "gt_prof_read", "gt_last_error", "gt_spectra", "gt_fluxes",
+    "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
+    "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
+    "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync",
 ]
 
 
@@ -73,6 +76,19 @@ def load():
     L.gt_prof_read.argtypes = [vp, vp, vp, vp, ip]
     L.gt_spectra.argtypes = [vp, vp, vp, vp, vp]
     L.gt_fluxes.argtypes = [vp, vp]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.gt_batch_create.argtypes = [ctypes.POINTER(GtParams), ip, vp, ctypes.POINTER(vp)]
+    L.gt_batch_size.argtypes = [vp, ctypes.POINTER(ip)]
+    L.gt_batch_set_state.argtypes = [vp, ip, vp, vp, vp]
+    L.gt_batch_get_state.argtypes = [vp, ip, vp, vp, vp]
+    L.gt_batch_step.argtypes = [vp, ip, i32p]
+    L.gt_batch_spectra.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.gt_ensemble_create.argtypes = [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(vp)]
+    L.gt_ensemble_destroy.argtypes = [vp]
+    L.gt_ensemble_set_seeds.argtypes = [vp, ip, i32p]
+    L.gt_ensemble_get_seeds.argtypes = [vp, ip, i32p]
+    L.gt_ensemble_step.argtypes = [vp, ip]
+    L.gt_ensemble_sync.argtypes = [vp]
     L.gt_last_error.restype = ctypes.c_char_p
     L.gt_last_error.argtypes = []
     _lib = L
@@ -230,3 +246,82 @@ class Context:
         by = np.zeros(NPASS)
         _check(load().gt_prof_read(self.h, _ptr(ms), _ptr(n), _ptr(by), NPASS), "gt_prof_read")
         return ms, n, by
+
+
+class Batch(Context):
+    """nmembers independent members on one grid (per-member omega and seed)."""
+
+    def __init__(self, p: Params, nmembers: int, omegas=None):
+        self.p = p
+        self.n = int(nmembers)
+        h = ctypes.c_void_p()
+        cp = to_c(p)
+        om = None if omegas is None else np.ascontiguousarray(omegas, dtype=np.float64)
+        if om is not None and om.shape != (self.n,):
+            raise ValueError("omegas must have one entry per member")
+        _check(load().gt_batch_create(ctypes.byref(cp), self.n, None if om is None else _ptr(om), ctypes.byref(h)),
+               "gt_batch_create")
+        self.h = h
+
+    def set_member(self, m: int, b) -> None:
+        b = self._fields(b, "set_member")
+        _check(load().gt_batch_set_state(self.h, int(m), *[_ptr(x) for x in b]), "gt_batch_set_state")
+
+    def get_member(self, m: int):
+        b = [np.empty(self.shape, np.complex128) for _ in range(3)]
+        _check(load().gt_batch_get_state(self.h, int(m), *[_ptr(x) for x in b]), "gt_batch_get_state")
+        return b
+
+    def step_batch(self, nsteps: int, seeds):
+        s = np.ascontiguousarray(seeds, dtype=np.int32).copy()
+        if s.shape != (self.n,):
+            raise ValueError("one seed per member")
+        _check(load().gt_batch_step(self.h, int(nsteps), s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "gt_batch_step")
+        return s
+
+    def member_spectra(self, m: int):
+        nb = self.p.nx // 2
+        sp, sp2d, hel, glob = np.zeros(nb), np.zeros(nb), np.zeros(nb), np.zeros(18)
+        _check(load().gt_batch_spectra(self.h, int(m), _ptr(sp), _ptr(sp2d), _ptr(hel), _ptr(glob)),
+               "gt_batch_spectra")
+        return sp, sp2d, hel, glob
+
+
+class Ensemble:
+    """Batches of several grids stepped together, chunk steps per hipGraph replay."""
+
+    def __init__(self, batches, chunk: int):
+        self.batches = list(batches)
+        arr = (ctypes.c_void_p * len(self.batches))(*[b.h.value for b in self.batches])
+        h = ctypes.c_void_p()
+        _check(load().gt_ensemble_create(arr, len(self.batches), int(chunk), ctypes.byref(h)), "gt_ensemble_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load().gt_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_seeds(self, q: int, seeds) -> None:
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        _check(load().gt_ensemble_set_seeds(self.h, int(q), s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "gt_ensemble_set_seeds")
+
+    def get_seeds(self, q: int):
+        s = np.zeros(self.batches[q].n, np.int32)
+        _check(load().gt_ensemble_get_seeds(self.h, int(q), s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "gt_ensemble_get_seeds")
+        return s
+
+    def step(self, nsteps: int) -> None:
+        _check(load().gt_ensemble_step(self.h, int(nsteps)), "gt_ensemble_step")
+
+    def sync(self) -> None:
+        _check(load().gt_ensemble_sync(self.h), "gt_ensemble_sync")

@@ -137,6 +137,30 @@ int gt_pruned(gt_ctx* c, int* pruned);
 int gt_prof_enable(gt_ctx* c, int on);
 int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int npass);
 
+/* Batches: nmembers independent members on one grid with the same
+ * parameters except omega (omegas[m], or p->omega when omegas is NULL) and
+ * the forcing seed.  Every pass covers all members in one launch.  No
+ * counterpart in the reference (one run per executable there). */
+int gt_batch_create(const gt_params* p, int nmembers, const double* omegas, gt_ctx** out);
+int gt_batch_size(gt_ctx* c, int* nmembers);
+int gt_batch_set_state(gt_ctx* c, int member, const double* bx, const double* by, const double* bz);
+int gt_batch_get_state(gt_ctx* c, int member, double* bx, double* by, double* bz);
+/* seeds[nmembers]: each member's LCG state, advanced on return */
+int gt_batch_step(gt_ctx* c, int nsteps, int32_t* seeds);
+int gt_batch_spectra(gt_ctx* c, int member, double* sp, double* sp2d, double* hel, double* glob);
+
+/* Ensembles: batches of different grids stepped together on one stream,
+ * chunk steps of all of them captured in hipGraphs and replayed.  Seeds are
+ * held by the ensemble (gt_ensemble_set_seeds / _get_seeds per batch).
+ * The batches stay owned by the caller and must outlive the ensemble. */
+typedef struct gt_ensemble gt_ensemble;
+int gt_ensemble_create(gt_ctx** batches, int nbatches, int chunk, gt_ensemble** out);
+int gt_ensemble_destroy(gt_ensemble* e);
+int gt_ensemble_set_seeds(gt_ensemble* e, int batch, const int32_t* seeds);
+int gt_ensemble_get_seeds(gt_ensemble* e, int batch, int32_t* seeds);
+int gt_ensemble_step(gt_ensemble* e, int nsteps);
+int gt_ensemble_sync(gt_ensemble* e);
+
 const char* gt_last_error(void);
 
 #ifdef __cplusplus

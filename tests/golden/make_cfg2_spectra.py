@@ -1,0 +1,65 @@
+"""Golden fixture for the long-run spectra check: the 256^3 rotating run
+(nu = 2e-4, Omega = 5, 2D2C forcing on (3, 4], dt = 2e-3, 2000 steps, seed 0)
+stepped by the CPU oracle, with the energy spectrum, the kz = 0 spectrum and
+glob recorded every 100 steps (imix = 100).
+
+    python tests/golden/make_cfg2_spectra.py [--steps 2000] [--threads 6]
+
+Writes tests/golden/cfg2_spectra_256.npz.  Takes about 1.5 h on 6 cores.
+"""
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from gturbo3d_amd import initial  # noqa: E402
+from gturbo3d_amd.params import FORCING_2D2C, Params  # noqa: E402
+import oracle_lib as ol  # noqa: E402
+
+
+def cfg2_params(n=256):
+    # famp = sqrt(0.1 / nf) with nf = 10 forced modes on (3, 4]
+    return Params(nx=n, ny=n, nz=n, nu=2e-4, alpnu=1.0, mu=0.0, alpmu=0.0, dt=2e-3, omega=5.0,
+                  rtrunc=2.0 / 3.0, truncate=1, famp=math.sqrt(0.1 / 10), k1f=3.0, k2f=4.0,
+                  variant=FORCING_2D2C, npas=2000, imix=100)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--imix", type=int, default=100)
+    ap.add_argument("--threads", type=int, default=6)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--out", default=os.path.join(HERE, "cfg2_spectra_256.npz"))
+    a = ap.parse_args()
+    ol.set_threads(a.threads)
+    p = cfg2_params(a.n)
+    o = ol.Oracle(p)
+    assert o.nf == 10, o.nf
+    b = initial.initial_state(p, seed=0)
+    seed = 0
+    sps, sp2ds, globs, steps = [], [], [], []
+    t0 = time.time()
+    for s in range(a.imix, a.steps + 1, a.imix):
+        b, seed = o.step(b, seed, a.imix)
+        sp, sp2d = o.spectrum(b)
+        sps.append(sp)
+        sp2ds.append(sp2d)
+        globs.append(o.glob(b))
+        steps.append(s)
+        print(f"step {s}: E = {sp.sum():.6f}  ({time.time() - t0:.0f} s)", flush=True)
+    np.savez_compressed(a.out, steps=np.array(steps), sp=np.array(sps), sp2d=np.array(sp2ds),
+                        glob=np.array(globs), seed=seed, n=a.n)
+    print("wrote", a.out)
+
+
+if __name__ == "__main__":
+    main()
