@@ -50,6 +50,51 @@ def cpu_baseline(p, b, seed, threads):
     return t1 - t0
 
 
+def ensemble_bench(args):
+    """32 members at 128^3 plus 16 at 96^3, FP64, nu = 1e-3, dt = 4e-3, 3D3C forcing (2,3]
+    rate 0.1, seeds 0..47, Omega ~ U[0.5, 10] from PCG64(48); chunk-step hipGraphs."""
+    import numpy as np
+    import torch
+    from gturbo3d_amd import initial, lib
+    from gturbo3d_amd.params import Params
+    rng = np.random.default_rng(48)
+    omegas = rng.uniform(0.5, 10.0, 48)
+    groups = [(128, 32, 0), (96, 16, 32)]
+    batches, tiles = [], {}
+    for n, nm, off in groups:
+        p = Params(nx=n, ny=n, nz=n, nu=1e-3, dt=4e-3, omega=1.0, famp=0.0, k1f=2.0, k2f=3.0)
+        p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
+        bt = lib.Batch(p, nm, omegas[off:off + nm])
+        for m in range(nm):
+            bt.set_member(m, initial.initial_state(p, seed=off + m))
+        tiles[f"{n}^3"] = dict(zip(("tz", "ty", "ms_per_nonlinear"), bt.autotune(3)))
+        batches.append(bt)
+    ens = lib.Ensemble(batches, chunk=10)
+    ens.set_seeds(0, np.arange(0, 32, dtype=np.int32))
+    ens.set_seeds(1, np.arange(32, 48, dtype=np.int32))
+    ens.step(args.warmup * 10)
+    ens.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.sync()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    pts = 32 * 128 ** 3 + 16 * 96 ** 3
+    finite = all(np.isfinite(f).all() for bt in batches for f in bt.get_member(0))
+    print(json.dumps({
+        "metric": "ensemble grid-pt·steps/s (32 x 128^3 + 16 x 96^3, FP64, one hipGraph per 10 steps)",
+        "value": pts * args.steps / t, "unit": "grid-pt·steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup * 10, "ms_per_step": 1e3 * t / args.steps, "member_steps_per_s": 48 * args.steps / t,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched ensemble", "members": {"128^3": 32, "96^3": 16},
+                   "omegas_pcg64_seed48": [round(float(o), 6) for o in omegas], "tiles": tiles},
+        "state_finite": bool(finite)}))
+    ens.close()
+    for bt in batches:
+        bt.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +104,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32-bulk"], default="fp64")
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["step", "copy"], default="step",
+    ap.add_argument("--mode", choices=["step", "copy", "ensemble"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
@@ -83,6 +128,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"mode": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
                           "launches": 5}))
+        return
+
+    if args.mode == "ensemble":
+        ensemble_bench(args)
         return
 
     n = args.size

@@ -63,6 +63,8 @@ struct gt_ctx {
   int prec = 0;
   int resident_blocks = 1024;
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
+  int tz = 0, ty = 0;                // pencils per block of the z / y passes (0: default)
+  double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
   // batches: nmem members with their own omega (and forcing seeds)
   int nmem = 1;
   std::vector<double> omegas;
@@ -324,6 +326,8 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.bstride = 3 * c->fs;
   kp.wstride = 6 * c->fs;
   kp.omega_m = c->nmem > 1 ? c->domega : nullptr;
+  kp.tz = c->tz;
+  kp.ty = c->ty;
   return kp;
 }
 
@@ -1002,6 +1006,74 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
 // ---------------------------------------------------------------------------
 // Batches and ensembles
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner
+// ---------------------------------------------------------------------------
+
+int gt_set_tiles(gt_ctx* c, int tz, int ty) {
+  if (int rc = check_ctx(c)) return rc;
+  auto ok = [](int n, int t) { return t == 0 || t == pencil_T(n) || (alt_T(n) > 0 && t == alt_T(n)); };
+  if (!ok(c->nz, tz) || !ok(c->ny, ty)) return fail(GT_EINVAL, "gt_set_tiles: no such tile width for this grid");
+  c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
+  c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
+  return GT_OK;
+}
+
+int gt_get_tiles(gt_ctx* c, int* tz, int* ty, double* ms) {
+  if (int rc = check_ctx(c)) return rc;
+  if (tz) *tz = c->tz ? c->tz : pencil_T(c->nz);
+  if (ty) *ty = c->ty ? c->ty : pencil_T(c->ny);
+  if (ms) *ms = c->tune_ms;
+  return GT_OK;
+}
+
+// Time nrep nonlinear evaluations of the current state (all members) for
+// every pencil width of the z and y passes and keep the fastest pair.  The
+// state is left unchanged; the widths do not change any result bit.
+int gt_autotune(gt_ctx* c, int nrep) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nrep < 1) return fail(GT_EINVAL, "gt_autotune: nrep must be >= 1");
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  std::vector<int> cz = {pencil_T(c->nz)}, cy = {pencil_T(c->ny)};
+  if (alt_T(c->nz)) cz.push_back(alt_T(c->nz));
+  if (alt_T(c->ny)) cy.push_back(alt_T(c->ny));
+  const bool prof = c->prof;
+  c->prof = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e30;
+  int btz = c->tz, bty = c->ty;
+  int rc = GT_OK;
+  for (int tz : cz)
+    for (int ty : cy) {
+      if (rc != GT_OK) break;
+      c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
+      c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
+      const KP kp = make_kp(c, false);
+      rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream);  // warm-up
+      if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune event");
+      for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream);
+      float ms = 0.f;
+      if (rc == GT_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(GT_EHIP, "autotune timing");
+      if (rc == GT_OK && ms / nrep < best) {
+        best = ms / nrep;
+        btz = c->tz;
+        bty = c->ty;
+      }
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->prof = prof;
+  c->tz = btz;
+  c->ty = bty;
+  c->tune_ms = best;
+  if (rc != GT_OK) return rc;
+  return set_mode(c, c->pruned);  // bh vanishes outside the kept extents again
+}
 
 int gt_batch_create(const gt_params* p, int nmembers, const double* omegas, gt_ctx** out) {
   return create_ctx(p, nmembers, omegas, out);

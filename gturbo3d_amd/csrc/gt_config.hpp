@@ -37,6 +37,12 @@ __host__ __device__ constexpr int pencil_P(int n) {
 #endif
 __host__ __device__ constexpr int pencil_T(int n) { return n >= 384 ? GT_PENCIL_T_LARGE : GT_PENCIL_T_SMALL; }
 
+// The other pencil count the autotuner may pick (0: none; the LDS tile
+// T*N*16 bytes stays at or under 128 KB).
+__host__ __device__ constexpr int alt_T(int n) {
+  return n > 512 ? 0 : (pencil_T(n) == 16 ? 8 : 16);
+}
+
 // x passes: threads per row sequence (three sequences per block)
 __host__ __device__ constexpr int row_P(int n) { return pencil_P(n); }
 

@@ -55,15 +55,23 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   }
 }
 
-template <typename W, int N>
-static hipError_t yfft_n(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
-  constexpr int T = pencil_T(N), P = pencil_P(N);
+template <typename W, int N, int T>
+static hipError_t yfft_nt(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
+  constexpr int P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.nz, nfields * kp.nmem);
   if (sign > 0)
     k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
   else
     k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
   return hipGetLastError();
+}
+
+template <typename W, int N>
+static hipError_t yfft_n(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
+  if constexpr (alt_T(N) > 0) {
+    if (kp.ty == alt_T(N)) return yfft_nt<W, N, alt_T(N)>(sign, nfields, kp, work, tw, s);
+  }
+  return yfft_nt<W, N, pencil_T(N)>(sign, nfields, kp, work, tw, s);
 }
 
 hipError_t launch_yfft(int prec, int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {

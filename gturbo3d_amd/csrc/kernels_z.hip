@@ -223,18 +223,26 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename W, int N>
-static hipError_t zinv_n(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
-  constexpr int T = pencil_T(N), P = pencil_P(N);
+template <typename W, int N, int T>
+static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
+  constexpr int P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.ny, nout * kp.nmem);
   k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
   return hipGetLastError();
 }
 
 template <typename W, int N>
-static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
-                         hipStream_t s) {
-  constexpr int T = pencil_T(N), P = pencil_P(N);
+static hipError_t zinv_n(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
+  if constexpr (alt_T(N) > 0) {
+    if (kp.tz == alt_T(N)) return zinv_nt<W, N, alt_T(N)>(nout, kp, b, work, tw, s);
+  }
+  return zinv_nt<W, N, pencil_T(N)>(nout, kp, b, work, tw, s);
+}
+
+template <typename W, int N, int T>
+static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
+                          hipStream_t s) {
+  constexpr int P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.ny, kp.nmem);
   const cx<W>* w = (const cx<W>*)work;
   const cx<double>* bb = (const cx<double>*)b;
@@ -248,6 +256,15 @@ static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b,
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <typename W, int N>
+static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
+                         hipStream_t s) {
+  if constexpr (alt_T(N) > 0) {
+    if (kp.tz == alt_T(N)) return zfwd_nt<W, N, alt_T(N)>(epi, kp, work, b, out, tw, s);
+  }
+  return zfwd_nt<W, N, pencil_T(N)>(epi, kp, work, b, out, tw, s);
 }
 
 hipError_t launch_zinv(int prec, int nout, const KP& kp, const void* b, void* work, const void* tw,

@@ -26,7 +26,8 @@ EXPORTS = [
     "gt_prof_enable", "gt_prof_read", "gt_last_error", "gt_spectra", "gt_fluxes",
     "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
-    "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync",
+    "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
+    "gt_get_tiles",
 ]
 
 
@@ -77,6 +78,9 @@ def load():
     L.gt_spectra.argtypes = [vp, vp, vp, vp, vp]
     L.gt_fluxes.argtypes = [vp, vp]
     i32p = ctypes.POINTER(ctypes.c_int32)
+    L.gt_autotune.argtypes = [vp, ip]
+    L.gt_set_tiles.argtypes = [vp, ip, ip]
+    L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_double)]
     L.gt_batch_create.argtypes = [ctypes.POINTER(GtParams), ip, vp, ctypes.POINTER(vp)]
     L.gt_batch_size.argtypes = [vp, ctypes.POINTER(ip)]
     L.gt_batch_set_state.argtypes = [vp, ip, vp, vp, vp]
@@ -226,6 +230,19 @@ class Context:
         spf = np.zeros(self.p.nx // 2)
         _check(load().gt_fluxes(self.h, _ptr(spf)), "gt_fluxes")
         return spf
+
+    def autotune(self, nrep: int = 3):
+        """Pick the fastest pencil widths; returns (tz, ty, ms per nonlinear evaluation)."""
+        _check(load().gt_autotune(self.h, int(nrep)), "gt_autotune")
+        return self.tiles()
+
+    def set_tiles(self, tz: int, ty: int) -> None:
+        _check(load().gt_set_tiles(self.h, int(tz), int(ty)), "gt_set_tiles")
+
+    def tiles(self):
+        tz, ty, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(load().gt_get_tiles(self.h, ctypes.byref(tz), ctypes.byref(ty), ctypes.byref(ms)), "gt_get_tiles")
+        return tz.value, ty.value, ms.value
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

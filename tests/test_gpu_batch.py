@@ -94,3 +94,26 @@ def test_ensemble_graph_replays_match_eager(require_gpu):
     ens.close()
     A.close()
     B.close()
+
+
+@pytest.mark.parametrize("n", [32, 96, 128])
+def test_tile_widths_do_not_change_results(require_gpu, n):
+    say(f"tile widths {n}^3")
+    p = params(n, omega=2.0)
+    b = initial.initial_state(p, seed=4)
+    results = {}
+    with lib.Context(p) as g:
+        g.set_state(b)
+        tz0, ty0, _ = g.tiles()
+        for tz in (8, 16):
+            for ty in (8, 16):
+                g.set_tiles(tz, ty)
+                g.set_state(b)
+                g.step(2, 3)
+                results[(tz, ty)] = g.get_state()
+        tz, ty, ms = g.autotune(2)
+        assert tz in (8, 16) and ty in (8, 16) and ms > 0
+    ref = results[(tz0, ty0)]
+    for key, st in results.items():
+        for x, y in zip(st, ref):
+            assert np.array_equal(x, y), key
