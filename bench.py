@@ -95,6 +95,88 @@ def ensemble_bench(args):
         bt.close()
 
 
+def adjoint_params():
+    from gturbo3d_amd import lib
+    from gturbo3d_amd.params import FORCING_3D3C, Params
+    p = Params(nx=128, ny=128, nz=128, nu=1e-3, dt=4e-3, omega=2.0, famp=0.0, k1f=2.0, k2f=3.0,
+               variant=FORCING_3D3C)
+    p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
+    return p
+
+
+def adjoint_bench(args):
+    """Gradient of the kinetic energy in 8 <= |k| <= 12 after 200 steps at
+    128^3 (FP64, nu = 1e-3, Omega = 2, 3D3C forcing (2,3] rate 0.1, seed 7,
+    dt = 4e-3) with respect to the initial vector potential and the per-shell
+    forcing amplitudes, checkpoints every 20 steps, through the
+    torch.autograd.Function.  --steps gradients are timed; the value is the
+    backward time (checkpoint recompute + adjoint) per step.  Central
+    differences on 16 random directions (eps = 1e-6, random-phase fields of
+    the initial spectrum) check the gradient."""
+    import numpy as np
+    import torch
+    from gturbo3d_amd import adjoint, initial, lib
+    p = adjoint_params()
+    nsteps, every, seed = 200, 20, 7
+    b0 = initial.initial_state(p, seed=0)
+    g = lib.Context(p)
+    g.set_state(b0)
+    tiles = g.autotune(3)
+    wt = torch.from_numpy(adjoint.band_weights(p, 8.0, 12.0))
+    nsh = len(g.forcing_shells())
+    t_fwd, t_bwd = [], []
+    for it in range(args.warmup + args.steps):
+        x = adjoint.to_real(b0).requires_grad_(True)
+        amps = torch.full((nsh,), p.famp, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        bN = adjoint.run(g, x, amps, nsteps, seed, every)
+        loss = adjoint.band_energy(bN, wt)
+        t1 = time.perf_counter()
+        loss.backward()
+        t2 = time.perf_counter()
+        if it >= args.warmup:
+            t_fwd.append(t1 - t0)
+            t_bwd.append(t2 - t1)
+    grad = adjoint.to_complex(x.grad)
+    ga = amps.grad.numpy().copy()
+    w = wt.numpy()[..., 0]
+
+    def J(b):
+        g.set_state(b)
+        g.step(nsteps, seed)
+        return sum(float(np.sum(w * np.abs(f) ** 2)) for f in g.get_state())
+
+    # random directions: random-phase fields of the initial spectrum
+    nb = math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in b0))
+    worst, eps = 0.0, 1e-6
+    t0 = time.perf_counter()
+    for k in range(16):
+        d = initial.initial_state(p, seed=100 + k)
+        sc = nb / math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in d))
+        d = [sc * f for f in d]
+        fd = (J([a + eps * c for a, c in zip(b0, d)]) - J([a - eps * c for a, c in zip(b0, d)])) / (2 * eps)
+        ad = sum(float(np.sum(np.real(np.conj(a) * c))) for a, c in zip(grad, d))
+        worst = max(worst, abs(ad - fd) / abs(fd))
+    t_fd = time.perf_counter() - t0
+    S = 16.0 * p.nz * p.ny * (p.nx // 2 + 1)
+    bwd = float(np.median(t_bwd))
+    print(json.dumps({
+        "metric": "adjoint step s (128^3 FP64, 200 steps, checkpoints every 20)",
+        "value": bwd / nsteps, "unit": "s per adjoint step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "forward_s_per_run": float(np.median(t_fwd)), "backward_s_per_run": bwd,
+        "gradient_s": float(np.median(t_fwd)) + bwd,
+        "model_bytes_per_adjoint_step": 270.0 * S,
+        "hbm_frac_model": 270.0 * S / (bwd / nsteps) / (HBM_PEAK_GBS * 1e9),
+        "fd_check": {"directions": 16, "eps": eps, "worst_rel_err": worst, "seconds": t_fd,
+                     "directions_from": "initial_state seeds 100..115 scaled to |b0|"},
+        "grad_amps": [float(v) for v in ga], "shells_k": [float(v) for v in g.forcing_shells()],
+        "config": {"workload": "adjoint 128^3", "nsteps": nsteps, "checkpoint_every": every, "seed": seed,
+                   "objective": "kinetic energy in 8 <= |k| <= 12 at the final time",
+                   "famp": p.famp, "tiles": dict(zip(("tz", "ty", "ms_per_nonlinear"), tiles))}}))
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,14 +186,14 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32-bulk"], default="fp64")
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["step", "copy", "ensemble"], default="step",
+    ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the single-GPU step only: this build has no multi-GPU path "
-              "(the slab decomposition with its all-to-all is a design in DESIGN.md). "
+              "(the slab decomposition with its all-to-all is a design, not code). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
@@ -132,6 +214,10 @@ def main():
 
     if args.mode == "ensemble":
         ensemble_bench(args)
+        return
+
+    if args.mode == "adjoint":
+        adjoint_bench(args)
         return
 
     n = args.size

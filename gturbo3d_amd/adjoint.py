@@ -1,0 +1,77 @@
+"""Differentiable runs of the step for PyTorch.
+
+``StepRun`` is a ``torch.autograd.Function`` whose forward runs nsteps steps
+of the library (checkpointing the state every ``every`` steps) and whose
+backward is the library's hand-written adjoint (gt_adjoint_forward /
+gt_adjoint_backward, through ctypes).  Inputs and outputs are float64
+tensors holding the real view of the three spectral fields,
+shape (3, NZ, NY, NX/2+1, 2), so torch's gradient of a real loss is
+(dJ/dRe, dJ/dIm) for every stored coefficient, and a float64 tensor of the
+per-shell forcing amplitudes.  The step itself has no counterpart in this
+form in the reference (src/main.f90:124-150 runs it forward only).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import lib
+from .params import Params
+
+
+def to_real(b) -> torch.Tensor:
+    """Three complex (NZ, NY, NXH) arrays -> float64 tensor (3, NZ, NY, NXH, 2)."""
+    a = np.stack([np.ascontiguousarray(f, dtype=np.complex128) for f in b])
+    return torch.from_numpy(a.view(np.float64).reshape(a.shape + (2,)).copy())
+
+
+def to_complex(t: torch.Tensor):
+    """float64 tensor (3, NZ, NY, NXH, 2) -> three complex128 arrays."""
+    a = np.ascontiguousarray(t.detach().to("cpu", torch.float64).numpy())
+    c = a.view(np.complex128)[..., 0]
+    return [np.ascontiguousarray(c[f]) for f in range(3)]
+
+
+class StepRun(torch.autograd.Function):
+    """b_N = run(b_0, amps): nsteps steps on context ``g`` from forcing seed ``seed``."""
+
+    @staticmethod
+    def forward(ctx, b0, amps, g, nsteps, seed, every):
+        g.set_state(to_complex(b0))
+        g.set_shell_amps(amps.detach().to("cpu", torch.float64).numpy())
+        ctx.seed_out = g.adjoint_forward(int(nsteps), int(seed), int(every))
+        ctx.g = g
+        return to_real(g.get_state()).to(b0.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        gb, ga = ctx.g.adjoint_backward(to_complex(grad))
+        dev = grad.device
+        return to_real(gb).to(dev), torch.from_numpy(ga.copy()).to(dev), None, None, None, None
+
+
+def run(g: lib.Context, b0: torch.Tensor, amps: torch.Tensor, nsteps: int, seed: int, every: int = 20):
+    """Differentiable b_N of nsteps steps (see StepRun)."""
+    return StepRun.apply(b0, amps, g, nsteps, seed, every)
+
+
+def band_weights(p: Params, kmin: float, kmax: float) -> np.ndarray:
+    """Weights w with sum(w * |b|^2) = energy in kmin <= |k| <= kmax of a
+    solenoidal b: fac * k^2, fac = 1/2 on the kx = 0 and kx = NX/2 planes
+    (the half-spectrum count of src/diagnostics.f90:22-35).  Shape
+    (NZ, NY, NXH, 1), for the real view."""
+    from .initial import wavenumbers
+
+    kx, ky, kz = wavenumbers(p)
+    k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    kk = np.sqrt(k2)
+    fac = np.ones(kx.shape)
+    fac[0] = 0.5
+    fac[-1] = 0.5
+    w = np.where((kk >= kmin) & (kk <= kmax), k2 * fac[None, None, :], 0.0)
+    return w[..., None]
+
+
+def band_energy(b: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    """J = sum_k w |b|^2 over the real view b (3, NZ, NY, NXH, 2)."""
+    return (w * b * b).sum()

@@ -17,7 +17,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgturbo3d.so")
 ROOT = os.path.dirname(PKG)
 
-SOURCES = ["kernels_z.hip", "kernels_y.hip", "kernels_x.hip", "kernels_diag.hip", "gt_api.cpp"]
+SOURCES = ["kernels_z.hip", "kernels_y.hip", "kernels_x.hip", "kernels_diag.hip", "kernels_adj.hip", "gt_api.cpp"]
 
 ARCH = os.environ.get("GT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -27,6 +27,14 @@ hipError_t launch_forcing(void* b, long long fs, const FEnt* ent, int n, bool se
 hipError_t launch_diag(const KP& kp, const void* b, const void* nl, double* partial, int nblocks, double* out,
                        int nbin, hipStream_t s);
 int diag_len(int nbin);
+hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, void* out, const void* tw, hipStream_t st);
+hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, hipStream_t st);
+hipError_t launch_xadj(const KP& kp, void* work, const void* tw, hipStream_t st);
+hipError_t launch_adj_ze(const KP& kp, const void* in, void* out, hipStream_t st);
+hipError_t launch_adj_post(const KP& kp, const void* nu, const void* g, void* lam_out, hipStream_t st);
+hipError_t launch_adj_stage1(const KP& kp, const void* h, const void* g1, void* acc, hipStream_t st);
+hipError_t launch_adj_fdot(const void* lam, long long fs, const FEnt* ent, int n, double* dots, hipStream_t st);
+bool adjoint_nx_supported(int n);
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 }  // namespace gt
 
@@ -79,6 +87,12 @@ struct gt_ctx {
   std::vector<Forced> forced;
   double inpe = 0, inpz = 0;
   bool force_seq = false;
+  // forcing shells: the distinct |k| of the forced modes, ascending; a
+  // per-shell amplitude replaces famp for the modes of that shell
+  std::vector<double> shell_k2;
+  std::vector<int> mode_shell;   // shell of each forced mode
+  std::vector<int> entry_shell;  // shell of each forcing entry of one step
+  std::vector<double> mode_amp;  // empty: famp for every mode
 
   // truncation extents
   int kxcut = 0;
@@ -105,6 +119,20 @@ struct gt_ctx {
   double* ddiag_part = nullptr;
   double* ddiag_out = nullptr;
   int diag_blocks = 0;
+
+  // adjoint: checkpoints of the forward run and the backward scratch
+  struct Adjoint {
+    int nsteps = 0, every = 0, nseg = 0;
+    bool ready = false;
+    std::vector<int32_t> seeds;     // forcing seed at the start of each segment
+    cx<double>* ckpt = nullptr;     // nseg + 1 states (the last one: the final state)
+    cx<double>* seg = nullptr;      // the states of one segment, every states
+    cx<double>* work = nullptr;     // 9 fields
+    cx<double>* vec = nullptr;      // 6 x 3 fields: lam nu g bh h g1
+    FEnt* dunit = nullptr;          // unit-amplitude forcing entries of one segment
+    double* ddots = nullptr;
+    size_t cap_ckpt = 0, cap_seg = 0, cap_unit = 0;
+  } adj;
 
   // profiling
   bool prof = false;
@@ -396,7 +424,9 @@ int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<do
 
 // forcing increments of one step (src/forcing_3d3c.f90:26-72,
 // src/forcing_2d2c.f90:120-135); appends entries and advances the seed
-void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out, int member = 0) {
+// unit: every amplitude 1 (the derivative of the increments with respect to
+// their shell's amplitude), else famp or the per-shell amplitudes
+void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out, int member = 0, bool unit = false) {
   auto rann = [](int32_t* s) -> float {  // src/forcing_3d3c.f90:82-93
     const uint32_t mask = 0x7FFFFFFFu;
     uint32_t v = (69069u * (uint32_t)(*s) + 1u) & mask;
@@ -404,9 +434,11 @@ void forcing_entries(const gt_ctx* c, int32_t* seed, FEnt* out, int member = 0) 
     return (float)((double)v / (double)mask);
   };
   const int nxh = c->nxh;
-  const double sf = c->sdt * c->p.famp;
   size_t o = 0;
-  for (const Forced& fm : c->forced) {
+  for (size_t im = 0; im < c->forced.size(); im++) {
+    const Forced& fm = c->forced[im];
+    const double amp = unit ? 1.0 : (c->mode_amp.empty() ? c->p.famp : c->mode_amp[im]);
+    const double sf = c->sdt * amp;
     const int i = fm.i, j = fm.j, k = fm.k;
     const long long mo = (long long)member * 3 * c->fs;
     const long long q = mo + elem_off(c, i - 1, j - 1, k - 1);
@@ -462,6 +494,30 @@ size_t entries_per_step(const gt_ctx* c) {
   return n;
 }
 
+// Forcing shells: forced modes grouped by k^2 (equal up to rounding), and
+// the shell of each entry forcing_entries emits for one step.
+void build_shells(gt_ctx* c) {
+  std::vector<double> k2s;
+  for (const Forced& f : c->forced) k2s.push_back(c->fkxs[f.i - 1] + c->fkys[f.j - 1] + c->fkzs[f.k - 1]);
+  std::vector<double> sorted = k2s;
+  std::sort(sorted.begin(), sorted.end());
+  c->shell_k2.clear();
+  for (double v : sorted)
+    if (c->shell_k2.empty() || v > c->shell_k2.back() * (1.0 + 1e-9)) c->shell_k2.push_back(v);
+  c->mode_shell.assign(c->forced.size(), 0);
+  c->entry_shell.clear();
+  for (size_t im = 0; im < c->forced.size(); im++) {
+    int sh = 0;
+    while (sh + 1 < (int)c->shell_k2.size() && k2s[im] > c->shell_k2[sh] * (1.0 + 1e-9)) sh++;
+    c->mode_shell[im] = sh;
+    const Forced& f = c->forced[im];
+    const bool conj = (c->p.variant == GT_FORCING_2D2C) ? (f.i == 1 || f.i == c->nxh)
+                                                          : ((f.i == 1 && f.j == 1) || f.i == 1 || f.i == c->nxh);
+    c->entry_shell.push_back(sh);
+    if (conj) c->entry_shell.push_back(sh);
+  }
+}
+
 void free_ctx(gt_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -483,6 +539,12 @@ void free_ctx(gt_ctx* c) {
     hipFree(c->tw32[a]);
   }
   hipFree(c->dforce);
+  hipFree(c->adj.ckpt);
+  hipFree(c->adj.seg);
+  hipFree(c->adj.work);
+  hipFree(c->adj.vec);
+  hipFree(c->adj.dunit);
+  hipFree(c->adj.ddots);
   hipFree(c->ddiag_part);
   hipFree(c->ddiag_out);
   if (c->hforce) hipHostFree(c->hforce);
@@ -564,6 +626,7 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
     return fail(GT_ENFORC, "gt_create: NFORC too small ! (more than 2048 forced modes)");
   }
   build_extents(c);
+  build_shells(c);
   {
     std::vector<long long> offs;
     for (const Forced& f : c->forced) {
@@ -668,6 +731,7 @@ static int set_state_m(gt_ctx* c, int member, const double* bx, const double* by
     HIPCHK(hipMemcpy2DAsync(dst + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
                             sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->stream));
   c->mem_inside[member] = inside ? 1 : 0;
+  c->adj.ready = false;
   bool all = true;
   for (char v : c->mem_inside) all = all && v;
   if (int rc = set_mode(c, all)) return rc;
@@ -720,12 +784,9 @@ static void fill_forcing(const gt_ctx* c, int nsteps, int32_t* seeds, FEnt* buf)
     for (int m = 0; m < c->nmem; m++) forcing_entries(c, &seeds[m], per ? buf + per * (c->nmem * s + m) : nullptr, m);
 }
 
-static int step_m(gt_ctx* c, int nsteps, int32_t* seeds) {
-  if (int rc = check_ctx(c)) return rc;
-  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
-  if (!seeds) return fail(GT_EINVAL, "step: null seed");
-  if (nsteps < 0) return fail(GT_EINVAL, "step: negative step count");
-  if (nsteps == 0) return GT_OK;
+// Compute the forcing of nsteps steps on the host and queue its upload to
+// dforce on the context stream (advancing seeds).
+static int stage_forcing(gt_ctx* c, int nsteps, int32_t* seeds) {
   const size_t per = entries_per_step(c);
   const size_t need = per * (size_t)c->nmem * (size_t)nsteps;
   if (need > 0) {
@@ -741,6 +802,16 @@ static int step_m(gt_ctx* c, int nsteps, int32_t* seeds) {
   } else {
     fill_forcing(c, nsteps, seeds, nullptr);
   }
+  return GT_OK;
+}
+
+static int step_m(gt_ctx* c, int nsteps, int32_t* seeds) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+  if (!seeds) return fail(GT_EINVAL, "step: null seed");
+  if (nsteps < 0) return fail(GT_EINVAL, "step: negative step count");
+  if (nsteps == 0) return GT_OK;
+  if (int rc = stage_forcing(c, nsteps, seeds)) return rc;
   return enqueue_steps(c, c->stream, nsteps, c->dforce);
 }
 
@@ -1266,6 +1337,227 @@ int gt_ensemble_step(gt_ensemble* e, int nsteps) {
 int gt_ensemble_sync(gt_ensemble* e) {
   if (!e) return fail(GT_EINVAL, "gt_ensemble_sync: null ensemble");
   HIPCHK(hipStreamSynchronize(e->stream));
+  return GT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Forcing shells and the adjoint of the step (FP64, single member).
+//
+// Forward: nsteps steps from the current state, the state and the forcing
+// seed kept every `every` steps.  Backward: for each segment, last first,
+// the states of the segment are recomputed from its checkpoint, then the
+// cotangent is carried back through the steps one at a time:
+//   dJ/dalpha_s += <lam_{n+1}, unit forcing increments of shell s at step n>
+//   lam_n = (d b_{n+1} / d b_n)^T lam_{n+1}
+// ---------------------------------------------------------------------------
+namespace {
+
+int adj_malloc(void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? GT_ENOMEM : GT_EHIP,
+                std::string("adjoint: ") + what + ": " + hipGetErrorString(e));
+  }
+  return GT_OK;
+}
+
+int adj_reserve(gt_ctx* c, size_t nckpt, size_t nseg_states, size_t nunit) {
+  gt_ctx::Adjoint& A = c->adj;
+  const size_t st = sizeof(cx<double>) * 3 * c->fs;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!A.work) {
+    if (int rc = adj_malloc((void**)&A.work, sizeof(cx<double>) * 9 * c->fs, "work fields")) return rc;
+    if (int rc = adj_malloc((void**)&A.vec, 6 * st, "cotangent fields")) return rc;
+  }
+  if (nckpt > A.cap_ckpt) {
+    hipFree(A.ckpt);
+    A.cap_ckpt = 0;
+    if (int rc = adj_malloc((void**)&A.ckpt, nckpt * st, "checkpoints")) return rc;
+    A.cap_ckpt = nckpt;
+  }
+  if (nseg_states > A.cap_seg) {
+    hipFree(A.seg);
+    A.cap_seg = 0;
+    if (int rc = adj_malloc((void**)&A.seg, nseg_states * st, "segment states")) return rc;
+    A.cap_seg = nseg_states;
+  }
+  if (nunit > A.cap_unit) {
+    hipFree(A.dunit);
+    hipFree(A.ddots);
+    A.ddots = nullptr;
+    A.cap_unit = 0;
+    if (int rc = adj_malloc((void**)&A.dunit, nunit * sizeof(FEnt), "forcing entries")) return rc;
+    if (int rc = adj_malloc((void**)&A.ddots, nunit * sizeof(double), "forcing dots")) return rc;
+    A.cap_unit = nunit;
+  }
+  return GT_OK;
+}
+
+// out = dN/da^T (s x) + nothing else, at a (one adjoint nonlinear evaluation)
+int adjoint_nlt(gt_ctx* c, const KP& kp, const cx<double>* a, const cx<double>* x, double s, cx<double>* out) {
+  gt_ctx::Adjoint& A = c->adj;
+  hipStream_t st = c->stream;
+  HIPCHK(launch_zinv(0, 6, kp, a, A.work, c->tw64[2], st));
+  HIPCHK(launch_zinv_adj(kp, x, s, A.work + 6 * c->fs, c->tw64[2], st));
+  HIPCHK(launch_yfft(0, +1, 9, kp, A.work, c->tw64[1], st));
+  HIPCHK(launch_xadj(kp, A.work, c->tw64[0], st));
+  HIPCHK(launch_yfft(0, -1, 6, kp, A.work, c->tw64[1], st));
+  HIPCHK(launch_zfwd_adj(kp, A.work, out, c->tw64[2], st));
+  return GT_OK;
+}
+
+// lam <- (d b_{n+1} / d b_n)^T lam, b_n = bn (the forcing does not depend on b)
+int step_adjoint(gt_ctx* c, const KP& kp, const cx<double>* bn) {
+  gt_ctx::Adjoint& A = c->adj;
+  hipStream_t st = c->stream;
+  const long long f3 = 3 * c->fs;
+  cx<double>* lam = A.vec;
+  cx<double>* nu = A.vec + f3;
+  cx<double>* g = A.vec + 2 * f3;
+  cx<double>* bh = A.vec + 3 * f3;
+  cx<double>* h = A.vec + 4 * f3;
+  cx<double>* g1 = A.vec + 5 * f3;
+  // step2: b+ = Z emk (emk b + dt N(bh; b))  (src/rkstep.f90:86-104)
+  HIPCHK(launch_adj_ze(kp, lam, nu, st));
+  // the stage-1 state the forward step used (src/rkstep.f90:30-48)
+  if (c->dt2 == 0.0) {
+    HIPCHK(launch_adj_ze(kp, bn, bh, st));
+  } else {
+    HIPCHK(hipMemcpyAsync(c->b, bn, sizeof(cx<double>) * f3, hipMemcpyDeviceToDevice, st));
+    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, bh, st)) return rc;
+  }
+  if (int rc = adjoint_nlt(c, kp, bh, nu, c->p.dt, g)) return rc;
+  HIPCHK(launch_adj_post(kp, nu, g, lam, st));
+  if (c->dt2 != 0.0) {
+    HIPCHK(launch_adj_ze(kp, g, h, st));
+    if (int rc = adjoint_nlt(c, kp, bn, h, c->dt2, g1)) return rc;
+    HIPCHK(launch_adj_stage1(kp, h, g1, lam, st));
+  }
+  return GT_OK;
+}
+
+int check_adjoint_ctx(gt_ctx* c, const char* what) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, std::string(what) + ": single-member contexts only");
+  if (c->prec != GT_PREC_FP64) return fail(GT_EINVAL, std::string(what) + ": FP64 contexts only");
+  if (!adjoint_nx_supported(c->nx))
+    return fail(GT_ESIZE, std::string(what) + ": NX = " + std::to_string(c->nx) + " not compiled (NX <= 512)");
+  return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gt_forcing_shells(gt_ctx* c, int* nshells, double* kk, int cap) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nshells) return fail(GT_EINVAL, "gt_forcing_shells: null count");
+  *nshells = (int)c->shell_k2.size();
+  if (kk)
+    for (int s = 0; s < std::min(cap, *nshells); s++) kk[s] = std::sqrt(c->shell_k2[s]);
+  return GT_OK;
+}
+
+int gt_set_shell_amps(gt_ctx* c, const double* amps, int nshells) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!amps) {
+    c->mode_amp.clear();
+  } else {
+    if (nshells != (int)c->shell_k2.size())
+      return fail(GT_EINVAL, "gt_set_shell_amps: expected " + std::to_string(c->shell_k2.size()) + " shells");
+    c->mode_amp.resize(c->forced.size());
+    for (size_t im = 0; im < c->forced.size(); im++) c->mode_amp[im] = amps[c->mode_shell[im]];
+  }
+  c->adj.ready = false;
+  return GT_OK;
+}
+
+int gt_adjoint_forward(gt_ctx* c, int nsteps, int32_t* seed, int every) {
+  if (int rc = check_adjoint_ctx(c, "gt_adjoint_forward")) return rc;
+  if (!seed) return fail(GT_EINVAL, "gt_adjoint_forward: null seed");
+  if (nsteps < 1 || every < 1) return fail(GT_EINVAL, "gt_adjoint_forward: need nsteps >= 1 and every >= 1");
+  gt_ctx::Adjoint& A = c->adj;
+  A.ready = false;
+  const int K = std::min(every, nsteps);
+  const int nseg = (nsteps + K - 1) / K;
+  const size_t per = entries_per_step(c);
+  if (int rc = adj_reserve(c, (size_t)nseg + 1, (size_t)K, std::max<size_t>(per * K, 1))) return rc;
+  const size_t st = sizeof(cx<double>) * 3 * c->fs;
+  A.seeds.assign(nseg, 0);
+  for (int s = 0; s < nseg; s++) {
+    HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)s * 3 * c->fs, c->b, st, hipMemcpyDeviceToDevice, c->stream));
+    A.seeds[s] = *seed;
+    if (int rc = step_m(c, std::min(K, nsteps - s * K), seed)) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)nseg * 3 * c->fs, c->b, st, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  A.nsteps = nsteps;
+  A.every = K;
+  A.nseg = nseg;
+  A.ready = true;
+  return GT_OK;
+}
+
+int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const double* lz, double* gx, double* gy,
+                        double* gz, double* grad_amps, int nshells) {
+  if (int rc = check_adjoint_ctx(c, "gt_adjoint_backward")) return rc;
+  gt_ctx::Adjoint& A = c->adj;
+  if (!A.ready) return fail(GT_EINVAL, "gt_adjoint_backward: no forward run to differentiate (gt_adjoint_forward)");
+  if (!lx || !ly || !lz || !gx || !gy || !gz) return fail(GT_EINVAL, "gt_adjoint_backward: null field");
+  if (grad_amps && nshells != (int)c->shell_k2.size())
+    return fail(GT_EINVAL, "gt_adjoint_backward: expected " + std::to_string(c->shell_k2.size()) + " shells");
+  hipStream_t st = c->stream;
+  const KP kp = make_kp1(c, false, 0);
+  const long long f3 = 3 * c->fs;
+  const size_t sbytes = sizeof(cx<double>) * f3;
+  const size_t per = entries_per_step(c);
+  const int K = A.every;
+  HIPCHK(hipMemsetAsync(A.work, 0, sizeof(cx<double>) * 9 * c->fs, st));
+  HIPCHK(hipMemsetAsync(A.vec, 0, 6 * sbytes, st));
+  const double* src[3] = {lx, ly, lz};
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpy2DAsync(A.vec + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
+                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, st));
+  std::vector<double> gsh(c->shell_k2.size(), 0.0);
+  std::vector<FEnt> hunit(std::max<size_t>(per * K, 1));
+  std::vector<double> hdots(std::max<size_t>(per * K, 1));
+  for (int s = A.nseg - 1; s >= 0; s--) {
+    const int len = std::min(K, A.nsteps - s * K);
+    // unit-amplitude increments of the segment's steps
+    int32_t sd = A.seeds[s];
+    for (int i = 0; i < len; i++) forcing_entries(c, &sd, per ? hunit.data() + per * i : nullptr, 0, true);
+    if (per) HIPCHK(hipMemcpyAsync(A.dunit, hunit.data(), sizeof(FEnt) * per * len, hipMemcpyHostToDevice, st));
+    // recompute the segment's states b_{sK} .. b_{sK+len-1}
+    int32_t sr = A.seeds[s];
+    HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)s * f3, sbytes, hipMemcpyDeviceToDevice, st));
+    if (len > 1)
+      if (int rc = stage_forcing(c, len - 1, &sr)) return rc;
+    for (int i = 0; i < len; i++) {
+      HIPCHK(hipMemcpyAsync(A.seg + (size_t)i * f3, c->b, sbytes, hipMemcpyDeviceToDevice, st));
+      if (i + 1 < len)
+        if (int rc = enqueue_steps(c, st, 1, c->dforce + per * i)) return rc;
+    }
+    for (int i = len - 1; i >= 0; i--) {
+      if (per) HIPCHK(launch_adj_fdot(A.vec, c->fs, A.dunit + per * i, (int)per, A.ddots + per * i, st));
+      if (int rc = step_adjoint(c, kp, A.seg + (size_t)i * f3)) return rc;
+    }
+    if (per) HIPCHK(hipMemcpyAsync(hdots.data(), A.ddots, sizeof(double) * per * len, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < len; i++)
+      for (size_t e = 0; e < per; e++) gsh[c->entry_shell[e]] += hdots[per * i + e];
+  }
+  double* dst[3] = {gx, gy, gz};
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, A.vec + f * c->fs, sizeof(cx<double>) * c->nxp,
+                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, st));
+  // leave the context at the forward run's final state
+  HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)A.nseg * f3, sbytes, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (grad_amps)
+    for (size_t q = 0; q < gsh.size(); q++) grad_amps[q] = gsh[q];
   return GT_OK;
 }
 

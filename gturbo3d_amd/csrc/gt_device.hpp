@@ -1,0 +1,56 @@
+// gt_device.hpp -- device helpers shared by the step kernels and the
+// adjoint kernels.
+#pragma once
+#include "gt_internal.hpp"
+
+namespace gt {
+
+// integrating factor exp(-0.5*dt*(nu*k2**alpnu + mu*k2**alpmu)), 1 at k = 0
+// (src/defk.f90:57-68), evaluated on the fly instead of read from a table
+__device__ __forceinline__ double emk_of(const KP& kp, double fk2) {
+  double scra = 0.0;
+  if (fk2 != 0.0) {
+    if (kp.visc_simple) scra = kp.nu * fk2;
+    else scra = kp.nu * pow(fk2, kp.alpnu) + kp.mu * pow(fk2, kp.alpmu);
+  }
+  double e = exp(-0.5 * kp.dt * scra);
+  if (fk2 == 0.0) e = 1.0;
+  return e;
+}
+
+// mode (kx, ky, kz) is zeroed by trunc: the ellipsoidal test with the sums in
+// the reference order, and k = 0 (src/trunc.f90:14-19,27); never when
+// truncate /= 1
+__device__ __forceinline__ bool trunc_zero(const KP& kp, int kx, int ky, int kz) {
+  if (kp.truncate != 1) return false;
+  if (kx == 0 && ky == 0 && kz == 0) return true;
+  return (kp.fkzn[kz] + kp.fkyn[ky]) + kp.fkxn[kx] > kp.rtrunc2;
+}
+
+// z = row0 + i*row1 packing of two half-spectrum rows for one complex
+// transform of length N (conj_half: element above N/2)
+template <typename W>
+__device__ __forceinline__ cx<W> herm_pack(cx<W> X0, cx<W> X1, bool conj_half) {
+  if (conj_half) return {X0.x + X1.y, -X0.y + X1.x};
+  return {X0.x - X1.y, X0.y + X1.x};
+}
+
+// Packed C2R input of one field at element n of the sequence: modes above
+// kc are zero, the imaginary parts of X[0] and X[N/2] are dropped (the C2R of
+// fft_inv, src/fft_inv.f90:12).
+template <typename W, int N>
+__device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n, int kc) {
+  const int mm = (n <= N / 2) ? n : N - n;
+  cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
+  if (mm <= kc) {
+    X0 = r0[mm];
+    X1 = r1[mm];
+  }
+  if (mm == 0 || mm == N / 2) {
+    X0.y = W(0);
+    X1.y = W(0);
+  }
+  return herm_pack(X0, X1, n > N / 2);
+}
+
+}  // namespace gt

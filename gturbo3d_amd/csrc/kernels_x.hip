@@ -15,31 +15,9 @@
 // k_xc2r / k_xr2c: plain transforms for the velocity output
 //   (src/main.f90:196-201) and the initial transform (src/main.f90:91-93).
 #include "gt_config.hpp"
-#include "gt_internal.hpp"
+#include "gt_device.hpp"
 
 namespace gt {
-
-template <typename W>
-__device__ __forceinline__ cx<W> herm_pack(cx<W> X0, cx<W> X1, bool conj_half) {
-  if (conj_half) return {X0.x + X1.y, -X0.y + X1.x};
-  return {X0.x - X1.y, X0.y + X1.x};
-}
-
-// Load the packed C2R input of one field at element n of the sequence.
-template <typename W, int N>
-__device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n, int kc) {
-  const int mm = (n <= N / 2) ? n : N - n;
-  cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
-  if (mm <= kc) {
-    X0 = r0[mm];
-    X1 = r1[mm];
-  }
-  if (mm == 0 || mm == N / 2) {
-    X0.y = W(0);
-    X1.y = W(0);
-  }
-  return herm_pack(X0, X1, n > N / 2);
-}
 
 // One block per row pair, six groups of P threads (one wavefront for
 // N <= 512).  Group w inverse-transforms field w in its own LDS row; with one

@@ -15,7 +15,7 @@
 // Both passes touch only the (kx, ky, kz) the truncation leaves alive in
 // pruned mode (see KP in gt_internal.hpp).
 #include "gt_config.hpp"
-#include "gt_internal.hpp"
+#include "gt_device.hpp"
 
 namespace gt {
 
@@ -91,18 +91,6 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
     if (F::out_ok(t, m))
 #pragma unroll
       for (int r = 0; r < RL; r++) dst[F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
-}
-
-__device__ __forceinline__ double emk_of(const KP& kp, double fk2) {
-  // src/defk.f90:61-65
-  double scra = 0.0;
-  if (fk2 != 0.0) {
-    if (kp.visc_simple) scra = kp.nu * fk2;
-    else scra = kp.nu * pow(fk2, kp.alpnu) + kp.mu * pow(fk2, kp.alpmu);
-  }
-  double e = exp(-0.5 * kp.dt * scra);
-  if (fk2 == 0.0) e = 1.0;
-  return e;
 }
 
 template <typename W, int N, int T, int P, int EPI>

@@ -27,7 +27,7 @@ EXPORTS = [
     "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
-    "gt_get_tiles",
+    "gt_get_tiles", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
 ]
 
 
@@ -93,6 +93,10 @@ def load():
     L.gt_ensemble_get_seeds.argtypes = [vp, ip, i32p]
     L.gt_ensemble_step.argtypes = [vp, ip]
     L.gt_ensemble_sync.argtypes = [vp]
+    L.gt_forcing_shells.argtypes = [vp, ctypes.POINTER(ip), vp, ip]
+    L.gt_set_shell_amps.argtypes = [vp, vp, ip]
+    L.gt_adjoint_forward.argtypes = [vp, ip, i32p, ip]
+    L.gt_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ip]
     L.gt_last_error.restype = ctypes.c_char_p
     L.gt_last_error.argtypes = []
     _lib = L
@@ -253,6 +257,40 @@ class Context:
         v = ctypes.c_int()
         _check(load().gt_pruned(self.h, ctypes.byref(v)), "gt_pruned")
         return bool(v.value)
+
+    def forcing_shells(self) -> np.ndarray:
+        """|k| of each forcing shell (forced modes grouped by |k|), ascending."""
+        n = ctypes.c_int(0)
+        _check(load().gt_forcing_shells(self.h, ctypes.byref(n), None, 0), "gt_forcing_shells")
+        kk = np.zeros(max(n.value, 1))
+        _check(load().gt_forcing_shells(self.h, ctypes.byref(n), _ptr(kk), n.value), "gt_forcing_shells")
+        return kk[: n.value]
+
+    def set_shell_amps(self, amps=None) -> None:
+        """Per-shell forcing amplitudes (None: famp for every shell)."""
+        if amps is None:
+            _check(load().gt_set_shell_amps(self.h, None, 0), "gt_set_shell_amps")
+            return
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        _check(load().gt_set_shell_amps(self.h, _ptr(a), int(a.size)), "gt_set_shell_amps")
+
+    def adjoint_forward(self, nsteps: int, seed: int, every: int) -> int:
+        """nsteps steps kept for the adjoint (a checkpoint every `every` steps);
+        returns the advanced seed."""
+        s = ctypes.c_int32(seed)
+        _check(load().gt_adjoint_forward(self.h, int(nsteps), ctypes.byref(s), int(every)), "gt_adjoint_forward")
+        return s.value
+
+    def adjoint_backward(self, lam):
+        """dJ/db_0 (three complex fields) and dJ/d(shell amplitudes) from
+        lam = dJ/db_N of the last adjoint_forward run."""
+        lam = self._fields(lam, "adjoint_backward")
+        g = [np.empty(self.shape, np.complex128) for _ in range(3)]
+        nsh = len(self.forcing_shells())
+        ga = np.zeros(max(nsh, 1))
+        _check(load().gt_adjoint_backward(self.h, *[_ptr(x) for x in lam], *[_ptr(x) for x in g], _ptr(ga), nsh),
+               "gt_adjoint_backward")
+        return g, ga[:nsh]
 
     def prof_enable(self, on: bool = True) -> None:
         _check(load().gt_prof_enable(self.h, 1 if on else 0), "gt_prof_enable")

@@ -171,6 +171,34 @@ int gt_ensemble_get_seeds(gt_ensemble* e, int batch, int32_t* seeds);
 int gt_ensemble_step(gt_ensemble* e, int nsteps);
 int gt_ensemble_sync(gt_ensemble* e);
 
+/* Forcing shells: the forced modes (src/iniforcing_3d3c.f90:26-40,
+ * src/iniforcing_2d2c.f90:12-68) grouped by |k|, ascending.  gt_forcing_shells
+ * returns their count and, when kk is not NULL, the first cap |k| values.
+ * gt_set_shell_amps replaces famp (src/forcing_3d3c.f90:34) by amps[s] for
+ * the modes of shell s (amps NULL: famp again); with every amps[s] = famp the
+ * step is bit-identical to the plain one.  No counterpart in the reference
+ * (one famp for all modes there). */
+int gt_forcing_shells(gt_ctx* c, int* nshells, double* kk, int cap);
+int gt_set_shell_amps(gt_ctx* c, const double* amps, int nshells);
+
+/* Adjoint of the step (no counterpart in the reference): reverse-mode
+ * derivatives of a run of the step of src/main.f90:124-150, FP64 contexts
+ * with one member and NX <= 512.
+ * gt_adjoint_forward runs nsteps steps from the current state like gt_step
+ * (advancing *seed) and keeps the state every `every` steps.
+ * gt_adjoint_backward takes lam = dJ/db_N of the final state (three fields
+ * in the gt_set_state layout, (re, im) = (dJ/dRe, dJ/dIm)) and returns
+ * dJ/db_0 in g[xyz] and, when grad_amps is not NULL, dJ/damps[s] for the
+ * nshells forcing shells (the derivative with respect to the per-shell
+ * amplitudes of gt_set_shell_amps, at the amplitudes of the forward run).
+ * It recomputes each segment of states from its checkpoint, may be called
+ * again with another lam, and leaves the context at the final state.  In
+ * pruned mode the derivatives are those of the map restricted to the modes
+ * trunc keeps (src/trunc.f90:10-28): components outside them read zero. */
+int gt_adjoint_forward(gt_ctx* c, int nsteps, int32_t* seed, int every);
+int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const double* lz, double* gx, double* gy,
+                        double* gz, double* grad_amps, int nshells);
+
 const char* gt_last_error(void);
 
 #ifdef __cplusplus
