@@ -9,6 +9,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gturbo3d.h"
@@ -27,12 +28,16 @@ hipError_t launch_forcing(void* b, long long fs, const FEnt* ent, int n, bool se
 hipError_t launch_diag(const KP& kp, const void* b, const void* nl, double* partial, int nblocks, double* out,
                        int nbin, hipStream_t s);
 int diag_len(int nbin);
-hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, void* out, const void* tw, hipStream_t st);
-hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, hipStream_t st);
+hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* tw, hipStream_t s);
+hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, bool ze, void* out, const void* tw,
+                           hipStream_t st);
+hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, const void* lam,
+                           hipStream_t st);
 hipError_t launch_xadj(const KP& kp, void* work, const void* tw, hipStream_t st);
 hipError_t launch_adj_ze(const KP& kp, const void* in, void* out, hipStream_t st);
 hipError_t launch_adj_post(const KP& kp, const void* nu, const void* g, void* lam_out, hipStream_t st);
 hipError_t launch_adj_stage1(const KP& kp, const void* h, const void* g1, void* acc, hipStream_t st);
+hipError_t launch_adj_project(const KP& kp, void* x, hipStream_t st);
 hipError_t launch_adj_fdot(const void* lam, long long fs, const FEnt* ent, int n, double* dots, hipStream_t st);
 bool adjoint_nx_supported(int n);
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
@@ -128,7 +133,7 @@ struct gt_ctx {
     cx<double>* ckpt = nullptr;     // nseg + 1 states (the last one: the final state)
     cx<double>* seg = nullptr;      // the states of one segment, every states
     cx<double>* work = nullptr;     // 9 fields
-    cx<double>* vec = nullptr;      // 6 x 3 fields: lam nu g bh h g1
+    cx<double>* vec = nullptr;      // 7 x 3 fields: lam (two, alternating) nu g bh h g1
     FEnt* dunit = nullptr;          // unit-amplitude forcing entries of one segment
     double* ddots = nullptr;
     size_t cap_ckpt = 0, cap_seg = 0, cap_unit = 0;
@@ -702,6 +707,30 @@ int gt_destroy(gt_ctx* c) {
   return GT_OK;
 }
 
+// Three host fields [nz][ny][nxh] <-> three pitched device fields: one linear
+// copy per field through a contiguous device staging area (3 nz ny nxh
+// complex) and one device-side 2D copy, instead of a 2D copy straight from
+// host memory, which moves one short row at a time.
+static int fields_to_device(gt_ctx* c, const double* const src[3], cx<double>* dst, void* staging, hipStream_t st) {
+  const size_t plane = (size_t)c->nz * c->ny * c->nxh;
+  cx<double>* stg = (cx<double>*)staging;
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpyAsync(stg + f * plane, src[f], sizeof(cx<double>) * plane, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(dst, sizeof(cx<double>) * c->nxp, stg, sizeof(cx<double>) * c->nxh,
+                          sizeof(cx<double>) * c->nxh, (size_t)3 * c->ny * c->nz, hipMemcpyDeviceToDevice, st));
+  return GT_OK;
+}
+
+static int fields_to_host(gt_ctx* c, const cx<double>* src, double* const dst[3], void* staging, hipStream_t st) {
+  const size_t plane = (size_t)c->nz * c->ny * c->nxh;
+  cx<double>* stg = (cx<double>*)staging;
+  HIPCHK(hipMemcpy2DAsync(stg, sizeof(cx<double>) * c->nxh, src, sizeof(cx<double>) * c->nxp,
+                          sizeof(cx<double>) * c->nxh, (size_t)3 * c->ny * c->nz, hipMemcpyDeviceToDevice, st));
+  for (int f = 0; f < 3; f++)
+    HIPCHK(hipMemcpyAsync(dst[f], stg + f * plane, sizeof(cx<double>) * plane, hipMemcpyDeviceToHost, st));
+  return GT_OK;
+}
+
 static int set_state_m(gt_ctx* c, int member, const double* bx, const double* by, const double* bz) {
   if (int rc = check_ctx(c)) return rc;
   if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
@@ -727,9 +756,8 @@ static int set_state_m(gt_ctx* c, int member, const double* bx, const double* by
   }
   cx<double>* dst = c->b + (size_t)member * 3 * c->fs;
   HIPCHK(hipMemsetAsync(dst, 0, sizeof(cx<double>) * 3 * c->fs, c->stream));
-  for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(dst + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
-                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, c->stream));
+  // the work fields (at least 3 fs complex doubles in either precision) stage the copy
+  if (int rc = fields_to_device(c, src, dst, c->work, c->stream)) return rc;
   c->mem_inside[member] = inside ? 1 : 0;
   c->adj.ready = false;
   bool all = true;
@@ -746,9 +774,7 @@ static int get_state_m(gt_ctx* c, int member, double* bx, double* by, double* bz
   if (member < 0 || member >= c->nmem) return fail(GT_EINVAL, "get_state: member out of range");
   double* dst[3] = {bx, by, bz};
   const cx<double>* src = c->b + (size_t)member * 3 * c->fs;
-  for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, src + f * c->fs, sizeof(cx<double>) * c->nxp,
-                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->stream));
+  if (int rc = fields_to_host(c, src, dst, c->work, c->stream)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return GT_OK;
 }
@@ -827,9 +853,7 @@ int gt_nonlinear(gt_ctx* c, double* nx, double* ny, double* nz) {
   const KP kp = make_kp(c, false);
   if (int rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream)) return rc;
   double* dst[3] = {nx, ny, nz};
-  for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, c->bh + f * c->fs, sizeof(cx<double>) * c->nxp,
-                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, c->stream));
+  if (int rc = fields_to_host(c, c->bh, dst, c->work, c->stream)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   // bh must vanish outside the kept extents before the next step
   if (int rc = set_mode(c, c->pruned)) return rc;
@@ -1370,7 +1394,7 @@ int adj_reserve(gt_ctx* c, size_t nckpt, size_t nseg_states, size_t nunit) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (!A.work) {
     if (int rc = adj_malloc((void**)&A.work, sizeof(cx<double>) * 9 * c->fs, "work fields")) return rc;
-    if (int rc = adj_malloc((void**)&A.vec, 6 * st, "cotangent fields")) return rc;
+    if (int rc = adj_malloc((void**)&A.vec, 7 * st, "cotangent fields")) return rc;
   }
   if (nckpt > A.cap_ckpt) {
     hipFree(A.ckpt);
@@ -1396,46 +1420,50 @@ int adj_reserve(gt_ctx* c, size_t nckpt, size_t nseg_states, size_t nunit) {
   return GT_OK;
 }
 
-// out = dN/da^T (s x) + nothing else, at a (one adjoint nonlinear evaluation)
-int adjoint_nlt(gt_ctx* c, const KP& kp, const cx<double>* a, const cx<double>* x, double s, cx<double>* out) {
+// The passes of one adjoint nonlinear evaluation after pass A: work fields
+// 0-5 hold pass A of the linearisation point; x is the cotangent (ze: nu is
+// Z emk x) scaled by s.  post == nullptr: out = dN/da^T (s nu); else out =
+// the step cotangent formed by k_zfwd_adj from post = lam and that g.
+int adjoint_nlt_tail(gt_ctx* c, const KP& kp, const cx<double>* x, double s, bool ze, cx<double>* out,
+                     const cx<double>* post) {
   gt_ctx::Adjoint& A = c->adj;
   hipStream_t st = c->stream;
-  HIPCHK(launch_zinv(0, 6, kp, a, A.work, c->tw64[2], st));
-  HIPCHK(launch_zinv_adj(kp, x, s, A.work + 6 * c->fs, c->tw64[2], st));
+  HIPCHK(launch_zinv_adj(kp, x, s, ze, A.work + 6 * c->fs, c->tw64[2], st));
   HIPCHK(launch_yfft(0, +1, 9, kp, A.work, c->tw64[1], st));
   HIPCHK(launch_xadj(kp, A.work, c->tw64[0], st));
   HIPCHK(launch_yfft(0, -1, 6, kp, A.work, c->tw64[1], st));
-  HIPCHK(launch_zfwd_adj(kp, A.work, out, c->tw64[2], st));
+  HIPCHK(launch_zfwd_adj(kp, A.work, out, c->tw64[2], post, st));
   return GT_OK;
 }
 
-// lam <- (d b_{n+1} / d b_n)^T lam, b_n = bn (the forcing does not depend on b)
-int step_adjoint(gt_ctx* c, const KP& kp, const cx<double>* bn) {
+// lam_out = (d b_{n+1} / d b_n)^T lam_in, b_n = bn (the forcing does not
+// depend on b).  step2: b+ = Z emk (emk b + dt N(bh; b)) (src/rkstep.f90:86-104),
+// stage 1: bh = Z emk (b + dt2 N(b; b)) (src/rkstep.f90:30-48).
+int step_adjoint(gt_ctx* c, const KP& kp, const cx<double>* bn, const cx<double>* lam_in, cx<double>* lam_out) {
   gt_ctx::Adjoint& A = c->adj;
   hipStream_t st = c->stream;
   const long long f3 = 3 * c->fs;
-  cx<double>* lam = A.vec;
-  cx<double>* nu = A.vec + f3;
-  cx<double>* g = A.vec + 2 * f3;
-  cx<double>* bh = A.vec + 3 * f3;
-  cx<double>* h = A.vec + 4 * f3;
-  cx<double>* g1 = A.vec + 5 * f3;
-  // step2: b+ = Z emk (emk b + dt N(bh; b))  (src/rkstep.f90:86-104)
-  HIPCHK(launch_adj_ze(kp, lam, nu, st));
-  // the stage-1 state the forward step used (src/rkstep.f90:30-48)
   if (c->dt2 == 0.0) {
-    HIPCHK(launch_adj_ze(kp, bn, bh, st));
-  } else {
-    HIPCHK(hipMemcpyAsync(c->b, bn, sizeof(cx<double>) * f3, hipMemcpyDeviceToDevice, st));
-    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, bh, st)) return rc;
+    // reference mode, fused: pass A of bh = Z emk bn, nu = Z emk lam on load,
+    // the whole step cotangent in the last z pass
+    HIPCHK(launch_zinv_emk(kp, bn, A.work, c->tw64[2], st));
+    return adjoint_nlt_tail(c, kp, lam_in, c->p.dt, true, lam_out, lam_in);
   }
-  if (int rc = adjoint_nlt(c, kp, bh, nu, c->p.dt, g)) return rc;
-  HIPCHK(launch_adj_post(kp, nu, g, lam, st));
-  if (c->dt2 != 0.0) {
-    HIPCHK(launch_adj_ze(kp, g, h, st));
-    if (int rc = adjoint_nlt(c, kp, bn, h, c->dt2, g1)) return rc;
-    HIPCHK(launch_adj_stage1(kp, h, g1, lam, st));
-  }
+  cx<double>* nu = A.vec + 2 * f3;
+  cx<double>* g = A.vec + 3 * f3;
+  cx<double>* bh = A.vec + 4 * f3;
+  cx<double>* h = A.vec + 5 * f3;
+  cx<double>* g1 = A.vec + 6 * f3;
+  HIPCHK(launch_adj_ze(kp, lam_in, nu, st));
+  HIPCHK(hipMemcpyAsync(c->b, bn, sizeof(cx<double>) * f3, hipMemcpyDeviceToDevice, st));
+  if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, bh, st)) return rc;
+  HIPCHK(launch_zinv(0, 6, kp, bh, A.work, c->tw64[2], st));
+  if (int rc = adjoint_nlt_tail(c, kp, nu, c->p.dt, false, g, nullptr)) return rc;
+  HIPCHK(launch_adj_post(kp, nu, g, lam_out, st));
+  HIPCHK(launch_adj_ze(kp, g, h, st));
+  HIPCHK(launch_zinv(0, 6, kp, bn, A.work, c->tw64[2], st));
+  if (int rc = adjoint_nlt_tail(c, kp, h, c->dt2, false, g1, nullptr)) return rc;
+  HIPCHK(launch_adj_stage1(kp, h, g1, lam_out, st));
   return GT_OK;
 }
 
@@ -1516,11 +1544,12 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   const size_t per = entries_per_step(c);
   const int K = A.every;
   HIPCHK(hipMemsetAsync(A.work, 0, sizeof(cx<double>) * 9 * c->fs, st));
-  HIPCHK(hipMemsetAsync(A.vec, 0, 6 * sbytes, st));
+  HIPCHK(hipMemsetAsync(A.vec, 0, 7 * sbytes, st));
+  cx<double>* lam = A.vec;          // the cotangent of the current state
+  cx<double>* lam_next = A.vec + f3;
   const double* src[3] = {lx, ly, lz};
-  for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(A.vec + f * c->fs, sizeof(cx<double>) * c->nxp, src[f], sizeof(double) * 2 * c->nxh,
-                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyHostToDevice, st));
+  if (int rc = fields_to_device(c, src, lam, A.work, st)) return rc;
+  if (kp.pruned) HIPCHK(launch_adj_project(kp, lam, st));
   std::vector<double> gsh(c->shell_k2.size(), 0.0);
   std::vector<FEnt> hunit(std::max<size_t>(per * K, 1));
   std::vector<double> hdots(std::max<size_t>(per * K, 1));
@@ -1541,8 +1570,9 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
         if (int rc = enqueue_steps(c, st, 1, c->dforce + per * i)) return rc;
     }
     for (int i = len - 1; i >= 0; i--) {
-      if (per) HIPCHK(launch_adj_fdot(A.vec, c->fs, A.dunit + per * i, (int)per, A.ddots + per * i, st));
-      if (int rc = step_adjoint(c, kp, A.seg + (size_t)i * f3)) return rc;
+      if (per) HIPCHK(launch_adj_fdot(lam, c->fs, A.dunit + per * i, (int)per, A.ddots + per * i, st));
+      if (int rc = step_adjoint(c, kp, A.seg + (size_t)i * f3, lam, lam_next)) return rc;
+      std::swap(lam, lam_next);
     }
     if (per) HIPCHK(hipMemcpyAsync(hdots.data(), A.ddots, sizeof(double) * per * len, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1550,9 +1580,7 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
       for (size_t e = 0; e < per; e++) gsh[c->entry_shell[e]] += hdots[per * i + e];
   }
   double* dst[3] = {gx, gy, gz};
-  for (int f = 0; f < 3; f++)
-    HIPCHK(hipMemcpy2DAsync(dst[f], sizeof(double) * 2 * c->nxh, A.vec + f * c->fs, sizeof(cx<double>) * c->nxp,
-                            sizeof(double) * 2 * c->nxh, (size_t)c->ny * c->nz, hipMemcpyDeviceToHost, st));
+  if (int rc = fields_to_host(c, lam, dst, A.work, st)) return rc;
   // leave the context at the forward run's final state
   HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)A.nseg * f3, sbytes, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));

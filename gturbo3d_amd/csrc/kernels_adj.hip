@@ -29,8 +29,9 @@
 
 namespace gt {
 
-// z stage of F^T applied to mu = I T (s * nu): output component q = blockIdx.z
-template <int N, int T, int P>
+// z stage of F^T applied to mu = I T (s * nu): output component q = blockIdx.z.
+// ZE: the input is lam and nu = Z emk lam is formed as it is loaded.
+template <int N, int T, int P, bool ZE>
 __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __restrict__ nu, double s,
                                                    cx<double>* __restrict__ out, const cx<double>* __restrict__ tw) {
   using F = Fft<double, N, P, 1, +1, LayInterleaved<T>>;
@@ -64,14 +65,17 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
       if (F::in_ok(t, m) && K >= 0 && keep_idx(n, N, K) && !trunc_zero(kp, kxc, j, n)) {
         const double fz = kp.fkz[n];
         const double k2 = fx * fx + fy * fy + fz * fz;
-        if (kxc == 0 && j == 0 && n == 0) {  // invcurl leaves k = 0 as it is
+        const bool k0 = kxc == 0 && j == 0 && n == 0;
+        const double e = ZE ? (k0 ? 0.0 : emk_of(kp, k2)) : 1.0;
+        if (k0) {  // invcurl leaves k = 0 as it is
           const cx<double> c = Q[0];
-          v = {wp * c.x, wp * c.y};
+          v = {wp * e * c.x, wp * e * c.y};
         } else {
           const cx<double> a = A[n * zs], bb = B[n * zs];
           const double ca = ((q == 0) ? fy : (q == 1) ? fz : fx) / k2;
           const double cb = ((q == 0) ? fz : (q == 1) ? fx : fy) / k2;
-          v = {wp * (-ca * a.y + cb * bb.y), wp * (ca * a.x - cb * bb.x)};
+          const double we = wp * e;
+          v = {we * (-ca * a.y + cb * bb.y), we * (ca * a.x - cb * bb.x)};
         }
       }
       in[0][m * R0 + r] = v;
@@ -167,10 +171,15 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
 }
 
 // z forward of A = Fu[w x p] (fields 0-2) and B = Fu[p x u] (3-5), then
-// g_q = (curl A)_q + k^2 B_q (wgt is already applied); component q = blockIdx.z
-template <int N, int T, int P>
+// g_q = (curl A)_q + k^2 B_q (wgt is already applied); component q = blockIdx.z.
+// POST (reference dt2 mode): instead of g, write the cotangent of the step's
+// input, out_q = emk nu_q - i omega kz (I T (dt nu))_q + Z emk g_q with
+// nu = Z emk lam (lam = the cotangent of the step's output; out must not
+// alias it: the three component blocks all read lam).
+template <int N, int T, int P, bool POST>
 __global__ __launch_bounds__(T* P) void k_zfwd_adj(KP kp, const cx<double>* __restrict__ work,
-                                                   cx<double>* __restrict__ out, const cx<double>* __restrict__ tw) {
+                                                   cx<double>* __restrict__ out, const cx<double>* __restrict__ tw,
+                                                   const cx<double>* __restrict__ lam) {
   using F = Fft<double, N, P, 1, -1, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<double> lds[N * T];
@@ -222,6 +231,26 @@ __global__ __launch_bounds__(T* P) void k_zfwd_adj(KP kp, const cx<double>* __re
       const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
       const cx<double> a = y[0][m * RL + r], bb = y[1][m * RL + r], cc = y[2][m * RL + r];
       cx<double> v = {-ca * a.y + cb * bb.y + k2 * cc.x, ca * a.x - cb * bb.x + k2 * cc.y};
+      if constexpr (POST) {
+        const bool k0 = kxc == 0 && j == 0 && n == 0;
+        const double e = emk_of(kp, k2);
+        const double ez = k0 ? 0.0 : e;
+        const int o = col + n * zs;
+        const cx<double> lq = lam[q * kp.fs + o], la = lam[fa * kp.fs + o], lb = lam[fb * kp.fs + o];
+        const cx<double> nq = {ez * lq.x, ez * lq.y};
+        // Coriolis: -i omega kz (I T (dt nu))_q, I = invcurl (k = 0 untouched)
+        cx<double> iv = {0.0, 0.0};
+        if (!trunc_zero(kp, kxc, j, n)) {
+          if (k0) {
+            iv = {kp.dt * nq.x, kp.dt * nq.y};
+          } else {
+            const double sa = kp.dt * ez * ca / k2, sb = kp.dt * ez * cb / k2;
+            iv = {-sa * la.y + sb * lb.y, sa * la.x - sb * lb.x};
+          }
+        }
+        const double of = kp.omega * fz;
+        v = {e * nq.x + of * iv.y + ez * v.x, e * nq.y - of * iv.x + ez * v.y};
+      }
       if (kp.pruned && trunc_zero(kp, kxc, j, n)) v = {0.0, 0.0};
       dst[n * zs] = v;
     }
@@ -339,6 +368,16 @@ __global__ void k_adj_stage1(KP kp, const cx<double>* __restrict__ h, const cx<d
   }
 }
 
+// x <- x with the modes trunc zeroes set to zero (pruned mode: the cotangent
+// of the final state restricted to the kept modes)
+__global__ void k_adj_project(KP kp, cx<double>* __restrict__ x) {
+  Mode md;
+  if (!mode_of(kp, (long long)blockIdx.x * blockDim.x + threadIdx.x, md)) return;
+  if (!trunc_zero(kp, md.i, md.j, md.k)) return;
+#pragma unroll
+  for (int f = 0; f < 3; f++) x[f * kp.fs + md.q] = {0.0, 0.0};
+}
+
 // dots[e] = sum_f Re(conj(lam_f[off_e]) d_f), the cotangent against one
 // forcing increment of unit amplitude
 __global__ void k_adj_fdot(const cx<double>* __restrict__ lam, long long fs, const FEnt* __restrict__ ent, int n,
@@ -359,26 +398,39 @@ __global__ void k_adj_fdot(const cx<double>* __restrict__ lam, long long fs, con
 // launchers
 // ---------------------------------------------------------------------------
 template <int N>
-static hipError_t zinv_adj_n(const KP& kp, const void* nu, double s, void* out, const void* tw, hipStream_t st) {
+static hipError_t zinv_adj_n(const KP& kp, const void* nu, double s, bool ze, void* out, const void* tw,
+                             hipStream_t st) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.ny, 3);
-  k_zinv_adj<N, T, P><<<grid, T * P, 0, st>>>(kp, (const cx<double>*)nu, s, (cx<double>*)out, (const cx<double>*)tw);
+  const cx<double>* x = (const cx<double>*)nu;
+  if (ze)
+    k_zinv_adj<N, T, P, true><<<grid, T * P, 0, st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
+  else
+    k_zinv_adj<N, T, P, false><<<grid, T * P, 0, st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
   return hipGetLastError();
 }
 
 template <int N>
-static hipError_t zfwd_adj_n(const KP& kp, const void* work, void* out, const void* tw, hipStream_t st) {
+static hipError_t zfwd_adj_n(const KP& kp, const void* work, void* out, const void* tw, const void* lam,
+                             hipStream_t st) {
   constexpr int T = pencil_T(N), P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.ny, 3);
-  k_zfwd_adj<N, T, P><<<grid, T * P, 0, st>>>(kp, (const cx<double>*)work, (cx<double>*)out, (const cx<double>*)tw);
+  const cx<double>* w = (const cx<double>*)work;
+  const cx<double>* t = (const cx<double>*)tw;
+  if (lam)
+    k_zfwd_adj<N, T, P, true><<<grid, T * P, 0, st>>>(kp, w, (cx<double>*)out, t, (const cx<double>*)lam);
+  else
+    k_zfwd_adj<N, T, P, false><<<grid, T * P, 0, st>>>(kp, w, (cx<double>*)out, t, nullptr);
   return hipGetLastError();
 }
 
-hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, void* out, const void* tw, hipStream_t st) {
+// ze: the input is lam and the kernel forms nu = Z emk lam
+hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, bool ze, void* out, const void* tw,
+                           hipStream_t st) {
   switch (kp.nz) {
 #define GT_CASE(N) \
   case N:          \
-    return zinv_adj_n<N>(kp, nu, s, out, tw, st);
+    return zinv_adj_n<N>(kp, nu, s, ze, out, tw, st);
     GT_FOR_EACH_N(GT_CASE)
 #undef GT_CASE
     default:
@@ -386,11 +438,13 @@ hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, void* out, co
   }
 }
 
-hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, hipStream_t st) {
+// lam == nullptr: out = g; else out = the step cotangent formed from lam and g
+hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, const void* lam,
+                           hipStream_t st) {
   switch (kp.nz) {
 #define GT_CASE(N) \
   case N:          \
-    return zfwd_adj_n<N>(kp, work, out, tw, st);
+    return zfwd_adj_n<N>(kp, work, out, tw, lam, st);
     GT_FOR_EACH_N(GT_CASE)
 #undef GT_CASE
     default:
@@ -444,6 +498,11 @@ hipError_t launch_adj_post(const KP& kp, const void* nu, const void* g, void* la
 
 hipError_t launch_adj_stage1(const KP& kp, const void* h, const void* g1, void* acc, hipStream_t st) {
   k_adj_stage1<<<mode_blocks(kp), 256, 0, st>>>(kp, (const cx<double>*)h, (const cx<double>*)g1, (cx<double>*)acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_project(const KP& kp, void* x, hipStream_t st) {
+  k_adj_project<<<mode_blocks(kp), 256, 0, st>>>(kp, (cx<double>*)x);
   return hipGetLastError();
 }
 

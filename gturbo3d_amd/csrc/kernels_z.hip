@@ -26,7 +26,9 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 // needs in registers.
 //   u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
 //   (src/produ.f90:48-53);  w_f = (fkxs + fkys + fkzs) b_f (src/nlt.f90:30-36)
-template <typename W, int N, int T, int P>
+// EMK: transform a = Z emk b instead of b (the stage-1 state bh of a step in
+// the reference dt2 mode, formed as it is loaded; Z sets k = 0 to zero)
+template <typename W, int N, int T, int P, bool EMK = false>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
                                                const cx<W>* __restrict__ tw, int nout) {
   using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
@@ -67,6 +69,12 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         if (keep_idx(n, N, K)) {
           va = A[n * zs];
           if (curl) vb = B[n * zs];
+          if constexpr (EMK) {
+            const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+            const double e = k0 ? 0.0 : emk_of(kp, fx * fx + fy * fy + fz * fz);
+            va = {e * va.x, e * va.y};
+            vb = {e * vb.x, e * vb.y};
+          }
         }
       }
       double re, im;
@@ -262,6 +270,24 @@ hipError_t launch_zinv(int prec, int nout, const KP& kp, const void* b, void* wo
   case N:                                                                  \
     return prec ? zinv_n<float, N>(nout, kp, b, work, tw, s)               \
                 : zinv_n<double, N>(nout, kp, b, work, tw, s);
+    GT_FOR_EACH_N(GT_CASE)
+#undef GT_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// pass A of a = Z emk b (FP64 work fields, default pencil width)
+hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
+  switch (kp.nz) {
+#define GT_CASE(N)                                                                                     \
+  case N: {                                                                                            \
+    constexpr int T = pencil_T(N), P = pencil_P(N);                                                    \
+    dim3 grid((kp.kxcut + T) / T, kp.ny, 6);                                                           \
+    k_zinv<double, N, T, P, true><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<double>*)work,  \
+                                                          (const cx<double>*)tw, 6);                   \
+    return hipGetLastError();                                                                          \
+  }
     GT_FOR_EACH_N(GT_CASE)
 #undef GT_CASE
     default:

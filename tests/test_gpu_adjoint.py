@@ -232,3 +232,27 @@ def test_adjoint_128_200_steps_16_directions(require_gpu):
             err = abs(dot(gg, d) - rich) / abs(dot(gg, d))
             say(f"  white direction, Richardson: relative error {err:.2e}")
             assert err < 1e-6
+
+
+def test_final_cotangent_is_projected_in_pruned_mode(require_gpu):
+    """Entries of lam_N on modes trunc removes do not reach the gradient, and
+    the gradient vanishes there (the state lives on the kept modes)."""
+    p = Params(nx=32, ny=32, nz=32, nu=2e-3, dt=5e-3, omega=2.0, famp=0.1, k1f=2.0, k2f=3.0)
+    say("pruned mode: lam_N projected on the kept modes")
+    b0 = initial.initial_state(p, seed=4)
+    w = band_w(p, 4.0, 8.0)
+    masked = initial.trunc_mask(p)
+    rng = np.random.default_rng(5)
+    with lib.Context(p) as g:
+        g.set_state(b0)
+        g.adjoint_forward(5, 9, 2)
+        bN = g.get_state()
+        lam = [2.0 * w * f for f in bN]
+        g1, a1 = g.adjoint_backward(lam)
+        noisy = [x + np.where(masked, rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape), 0.0)
+                 for x in lam]
+        g2, a2 = g.adjoint_backward(noisy)
+    for u, v in zip(g1, g2):
+        assert np.array_equal(u, v)
+        assert not np.any(u[masked])
+    assert np.array_equal(a1, a2)
