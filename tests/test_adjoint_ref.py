@@ -109,3 +109,38 @@ def test_transposed_transforms():
     lhs = float(np.sum(np.real(np.conj(m) * o.fft_dir(g))))
     rhs = float(np.sum(g * ref.finv(m / ref.wx) / ref.ntot))
     assert lhs == pytest.approx(rhs, rel=1e-13)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_numpy_adjoint_matches_torch_autograd(name):
+    """The hand derivation against torch's autograd through a PyTorch
+    restatement of the step (tests/torch_step.py); in truncated runs the
+    comparison is on the kept modes (the projection of torch's gradient)."""
+    import torch
+    from torch_step import TorchStep
+
+    p = case_params(CASES[name])
+    o = ol.Oracle(p)
+    ref = AdjointRef(p, o)
+    ts = TorchStep(p, o)
+    b0 = initial.initial_state(p, seed=5, energy=0.5, k0=2.0)
+    w = band_w(p, 2.0, 5.0)
+    sh = ref.shells()
+    x = torch.from_numpy(np.stack(b0).view(np.float64).reshape((3,) + b0[0].shape + (2,)).copy())
+    x.requires_grad_(True)
+    amps = torch.full((len(sh),), p.famp, dtype=torch.float64, requires_grad=True)
+    bN = ts.run(x, amps, 17, 3, sh)
+    obN, _ = o.step(b0, 17, 3)
+    tb = [bN.detach().numpy()[f].view(np.complex128)[..., 0] for f in range(3)]
+    assert math.sqrt(dot([a - b for a, b in zip(tb, obN)], [a - b for a, b in zip(tb, obN)])) \
+        < 1e-12 * math.sqrt(dot(obN, obN))
+    (torch.from_numpy(w)[..., None] * bN * bN).sum().backward()
+    g0, ga = ref.gradient(b0, 17, 3, [2.0 * w * f for f in obN])
+    gt = [x.grad.numpy()[f].view(np.complex128)[..., 0] for f in range(3)]
+    if p.truncate == 1:
+        keep = ~initial.trunc_mask(p)
+        keep[0, 0, 0] = False
+        gt = [np.where(keep, f, 0.0) for f in gt]
+    diff = [a - b for a, b in zip(g0, gt)]
+    assert math.sqrt(dot(diff, diff)) < 1e-11 * math.sqrt(dot(gt, gt))
+    assert np.allclose(ga, amps.grad.numpy(), rtol=1e-10, atol=0)
