@@ -67,7 +67,7 @@ def ensemble_bench(args):
         bt = lib.Batch(p, nm, omegas[off:off + nm])
         for m in range(nm):
             bt.set_member(m, initial.initial_state(p, seed=off + m))
-        tiles[f"{n}^3"] = dict(zip(("tz", "ty", "ms_per_nonlinear"), bt.autotune(3)))
+        tiles[f"{n}^3"] = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), bt.autotune(3)))
         batches.append(bt)
     ens = lib.Ensemble(batches, chunk=10)
     ens.set_seeds(0, np.arange(0, 32, dtype=np.int32))
@@ -173,7 +173,7 @@ def adjoint_bench(args):
         "grad_amps": [float(v) for v in ga], "shells_k": [float(v) for v in g.forcing_shells()],
         "config": {"workload": "adjoint 128^3", "nsteps": nsteps, "checkpoint_every": every, "seed": seed,
                    "objective": "kinetic energy in 8 <= |k| <= 12 at the final time",
-                   "famp": p.famp, "tiles": dict(zip(("tz", "ty", "ms_per_nonlinear"), tiles))}}))
+                   "famp": p.famp, "tiles": dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), tiles))}}))
     g.close()
 
 
@@ -186,6 +186,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32-bulk"], default="fp64")
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
     ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
@@ -232,6 +233,10 @@ def main():
     torch.cuda.init()
     ctx = lib.Context(p)
     ctx.set_state(b)
+    tiles = None
+    if not args.no_autotune:
+        # pencil widths of the z-inverse, z-forward and y passes (bit-neutral)
+        tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.autotune(3)))
     seed = seed0
     seed = ctx.step(args.warmup, seed)
     ctx.sync()
@@ -304,6 +309,7 @@ def main():
                         f"dt=1e-3, {args.precision}",
             "grid": [n, n, n],
             "pruned": pruned,
+            "tiles": tiles,
         },
         "hbm_frac_dense_model": dense_bytes * args.steps / t / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_step": alg_bytes_step,

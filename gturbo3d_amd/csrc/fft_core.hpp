@@ -17,7 +17,8 @@
 //
 // Twiddles come from a host-built table (long-double sin/cos rounded once);
 // stage s uses entries [off(s) + k*(R-1) + (r-1)] = exp(-2*pi*i*r*k/(Ns*R)),
-// conjugated for the inverse transform.
+// conjugated for the inverse transform.  FP64 transforms read only the r = 1
+// entry and form the powers by multiplication.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -275,16 +276,34 @@ struct Fft {
         if constexpr (s > 0) {
           const int k = j % Ns;
           const cx<T>* w = tw + PL::twoff(s) + k * (R - 1);
-          cx<T> wr[R - 1];
-#pragma unroll
-          for (int r = 1; r < R; r++) wr[r - 1] = w[r - 1];
-#pragma unroll
-          for (int g = 0; g < G; g++)
+          if constexpr (sizeof(T) == 8) {
+            // FP64: w^r = w^(r-1) w from the first table entry, applied as it
+            // is formed, so two twiddles are live instead of R-1 (register
+            // pressure sets the occupancy of the fused x pass); the products
+            // add at most R-2 roundings to w^r
+            const cx<T> w1 = w[0];
+            cx<T> wk = w1;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-              if constexpr (SIGN < 0) v[g][m * R + r] = cmul(v[g][m * R + r], wr[r - 1]);
-              else v[g][m * R + r] = cmulc(v[g][m * R + r], wr[r - 1]);
+              if (r > 1) wk = cmul(wk, w1);
+#pragma unroll
+              for (int g = 0; g < G; g++) {
+                if constexpr (SIGN < 0) v[g][m * R + r] = cmul(v[g][m * R + r], wk);
+                else v[g][m * R + r] = cmulc(v[g][m * R + r], wk);
+              }
             }
+          } else {
+            cx<T> wr[R - 1];
+#pragma unroll
+            for (int r = 1; r < R; r++) wr[r - 1] = w[r - 1];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+              for (int r = 1; r < R; r++) {
+                if constexpr (SIGN < 0) v[g][m * R + r] = cmul(v[g][m * R + r], wr[r - 1]);
+                else v[g][m * R + r] = cmulc(v[g][m * R + r], wr[r - 1]);
+              }
+          }
         }
 #pragma unroll
         for (int g = 0; g < G; g++) dft<R, SIGN>(&v[g][m * R]);

@@ -76,7 +76,7 @@ struct gt_ctx {
   int prec = 0;
   int resident_blocks = 1024;
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
-  int tz = 0, ty = 0;                // pencils per block of the z / y passes (0: default)
+  int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
   // batches: nmem members with their own omega (and forcing seeds)
   int nmem = 1;
@@ -361,6 +361,7 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.omega_m = c->nmem > 1 ? c->domega : nullptr;
   kp.tz = c->tz;
   kp.ty = c->ty;
+  kp.tze = c->tze;
   return kp;
 }
 
@@ -1106,66 +1107,86 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
 // Launch-shape autotuner
 // ---------------------------------------------------------------------------
 
-int gt_set_tiles(gt_ctx* c, int tz, int ty) {
+int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty) {
   if (int rc = check_ctx(c)) return rc;
   auto ok = [](int n, int t) { return t == 0 || t == pencil_T(n) || (alt_T(n) > 0 && t == alt_T(n)); };
-  if (!ok(c->nz, tz) || !ok(c->ny, ty)) return fail(GT_EINVAL, "gt_set_tiles: no such tile width for this grid");
+  if (!ok(c->nz, tz) || !ok(c->nz, tze) || !ok(c->ny, ty))
+    return fail(GT_EINVAL, "gt_set_tiles: no such tile width for this grid");
   c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
+  c->tze = (tze == pencil_T(c->nz)) ? 0 : tze;
   c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
   return GT_OK;
 }
 
-int gt_get_tiles(gt_ctx* c, int* tz, int* ty, double* ms) {
+int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms) {
   if (int rc = check_ctx(c)) return rc;
   if (tz) *tz = c->tz ? c->tz : pencil_T(c->nz);
+  if (tze) *tze = c->tze ? c->tze : pencil_T(c->nz);
   if (ty) *ty = c->ty ? c->ty : pencil_T(c->ny);
   if (ms) *ms = c->tune_ms;
   return GT_OK;
 }
 
-// Time nrep nonlinear evaluations of the current state (all members) for
-// every pencil width of the z and y passes and keep the fastest pair.  The
-// state is left unchanged; the widths do not change any result bit.
+// Each width acts on its own passes only (tz: pass A, tze: pass E, ty:
+// passes B and D), so the tuner times nrep nonlinear evaluations of the
+// current state (all members) with every width at each candidate, with an
+// event pair around every pass launch, and picks each width from the time
+// of its own passes.  The state is left unchanged; the widths do not change
+// any result bit.
 int gt_autotune(gt_ctx* c, int nrep) {
   if (int rc = check_ctx(c)) return rc;
   if (nrep < 1) return fail(GT_EINVAL, "gt_autotune: nrep must be >= 1");
   if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
-  std::vector<int> cz = {pencil_T(c->nz)}, cy = {pencil_T(c->ny)};
-  if (alt_T(c->nz)) cz.push_back(alt_T(c->nz));
-  if (alt_T(c->ny)) cy.push_back(alt_T(c->ny));
+  // only lengths with a second width for both axes have a choice
+  std::vector<int> cand = {8, 16};
   const bool prof = c->prof;
-  c->prof = false;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  double best = 1e30;
-  int btz = c->tz, bty = c->ty;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<gt_ctx::Rec> saved;
+  saved.swap(c->recs);
+  double tA[2] = {1e30, 1e30}, tE[2] = {1e30, 1e30}, tY[2] = {1e30, 1e30}, tall[2] = {0, 0};
   int rc = GT_OK;
-  for (int tz : cz)
-    for (int ty : cy) {
-      if (rc != GT_OK) break;
-      c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
-      c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
-      const KP kp = make_kp(c, false);
-      rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream);  // warm-up
-      if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune event");
-      for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_NONLIN, c->bh, c->stream);
-      float ms = 0.f;
-      if (rc == GT_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        rc = fail(GT_EHIP, "autotune timing");
-      if (rc == GT_OK && ms / nrep < best) {
-        best = ms / nrep;
-        btz = c->tz;
-        bty = c->ty;
-      }
+  auto ok = [](int n, int t) { return t == pencil_T(n) || (alt_T(n) > 0 && t == alt_T(n)); };
+  for (int ci = 0; ci < 2 && rc == GT_OK; ci++) {
+    const int w = cand[ci];
+    c->tz = c->tze = ok(c->nz, w) ? (w == pencil_T(c->nz) ? 0 : w) : 0;
+    c->ty = ok(c->ny, w) ? (w == pencil_T(c->ny) ? 0 : w) : 0;
+    const KP kp = make_kp(c, false);
+    c->prof = false;
+    rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
+    c->prof = true;
+    for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);
+    c->prof = false;
+    if (rc == GT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune sync");
+    double ms[NPASS] = {0};
+    for (auto& q : c->recs) {
+      float t = 0.f;
+      if (rc == GT_OK && hipEventElapsedTime(&t, q.a, q.b) != hipSuccess) rc = fail(GT_EHIP, "autotune timing");
+      ms[q.pass] += t;
+      c->pool.push_back(q.a);
+      c->pool.push_back(q.b);
     }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+    c->recs.clear();
+    if (ok(c->nz, w)) {
+      tA[ci] = ms[PASS_A] / nrep;
+      tE[ci] = ms[PASS_E1] / nrep;
+    }
+    if (ok(c->ny, w)) tY[ci] = (ms[PASS_B] + ms[PASS_D]) / nrep;
+    for (int q = 0; q < NPASS; q++) tall[ci] += ms[q] / nrep;
+  }
+  c->recs.swap(saved);
   c->prof = prof;
-  c->tz = btz;
-  c->ty = bty;
-  c->tune_ms = best;
+  auto pick = [&](const double* t, int n) {
+    const int w = (t[1] < t[0]) ? cand[1] : cand[0];
+    return (w == pencil_T(n) || !(alt_T(n) > 0 && w == alt_T(n))) ? 0 : w;
+  };
+  c->tz = pick(tA, c->nz);
+  c->tze = pick(tE, c->nz);
+  c->ty = pick(tY, c->ny);
+  // the tuned time per evaluation: the chosen passes plus the others (pass
+  // C, forcing) as timed with width 8, which every length has
+  const int iz = (c->tz ? c->tz : pencil_T(c->nz)) == cand[1], ie = (c->tze ? c->tze : pencil_T(c->nz)) == cand[1];
+  const int iy = (c->ty ? c->ty : pencil_T(c->ny)) == cand[1];
+  c->tune_ms = tA[iz] + tE[ie] + tY[iy] + (tall[0] - tA[0] - tE[0] - tY[0]);
   if (rc != GT_OK) return rc;
   return set_mode(c, c->pruned);  // bh vanishes outside the kept extents again
 }

@@ -37,18 +37,32 @@ __device__ __forceinline__ cx<W> herm_pack(cx<W> X0, cx<W> X1, bool conj_half) {
 
 // Packed C2R input of one field at element n of the sequence: modes above
 // kc are zero, the imaginary parts of X[0] and X[N/2] are dropped (the C2R of
-// fft_inv, src/fft_inv.f90:12).
+// fft_inv, src/fft_inv.f90:12).  For FP64 rows the loads are unconditional
+// (a lane above kc reads element 0 of its row and discards it), so that a
+// thread issues all its loads before it waits for any: a load under a
+// branch is waited for inside the branch.
 template <typename W, int N>
 __device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n, int kc) {
   const int mm = (n <= N / 2) ? n : N - n;
+  const bool ok = mm <= kc;
+  const bool edge = (mm == 0 || mm == N / 2);
   cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
-  if (mm <= kc) {
-    X0 = r0[mm];
-    X1 = r1[mm];
-  }
-  if (mm == 0 || mm == N / 2) {
-    X0.y = W(0);
-    X1.y = W(0);
+  if constexpr (sizeof(W) == 8) {
+    const int at = ok ? mm : 0;
+    X0 = r0[at];
+    X1 = r1[at];
+    X0 = {ok ? X0.x : W(0), (ok && !edge) ? X0.y : W(0)};
+    X1 = {ok ? X1.x : W(0), (ok && !edge) ? X1.y : W(0)};
+  } else {
+    // FP32 rows: occupancy hides the latency; predicated loads fetch less
+    if (ok) {
+      X0 = r0[mm];
+      X1 = r1[mm];
+    }
+    if (edge) {
+      X0.y = W(0);
+      X1.y = W(0);
+    }
   }
   return herm_pack(X0, X1, n > N / 2);
 }

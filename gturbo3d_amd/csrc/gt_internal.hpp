@@ -43,7 +43,8 @@ struct KP {
   int nmem;
   long long bstride, wstride;
   const double* omega_m;  // per-member omega, or nullptr for omega
-  int tz, ty;  // pencils per block of the z and y passes (0: the default)
+  int tz, ty;  // pencils per block of the z-inverse and y passes (0: the default)
+  int tze;     // pencils per block of the z-forward pass (0: the default)
 };
 
 __device__ __forceinline__ bool keep_idx(int n, int N, int K) {

@@ -56,6 +56,19 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
   const cx<double>* B = nu + fb * kp.fs + col;
   const cx<double>* Q = nu + q * kp.fs + col;
   cx<double> in[1][M0 * R0], o[1][ML * RL];
+  // loads first and unconditional (element 0 of the column for an unused
+  // slot), so that they are all in flight before the first wait
+  cx<double> la[M0 * R0], lb[M0 * R0];
+#pragma unroll
+  for (int m = 0; m < M0; m++)
+#pragma unroll
+    for (int r = 0; r < R0; r++) {
+      const int n = F::in_pos(t, m, r);
+      const int at = (F::in_ok(t, m) && K >= 0 && keep_idx(n, N, K)) ? n : 0;
+      la[m * R0 + r] = A[at * zs];
+      lb[m * R0 + r] = B[at * zs];
+    }
+  const cx<double> c0 = Q[0];
 #pragma unroll
   for (int m = 0; m < M0; m++)
 #pragma unroll
@@ -68,10 +81,9 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
         const bool k0 = kxc == 0 && j == 0 && n == 0;
         const double e = ZE ? (k0 ? 0.0 : emk_of(kp, k2)) : 1.0;
         if (k0) {  // invcurl leaves k = 0 as it is
-          const cx<double> c = Q[0];
-          v = {wp * e * c.x, wp * e * c.y};
+          v = {wp * e * c0.x, wp * e * c0.y};
         } else {
-          const cx<double> a = A[n * zs], bb = B[n * zs];
+          const cx<double> a = la[m * R0 + r], bb = lb[m * R0 + r];
           const double ca = ((q == 0) ? fy : (q == 1) ? fz : fx) / k2;
           const double cb = ((q == 0) ? fz : (q == 1) ? fx : fy) / k2;
           const double we = wp * e;

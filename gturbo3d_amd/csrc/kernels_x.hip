@@ -68,6 +68,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   // u x w (src/produ.f90:14-16 with a = u, b = w), component w, both rows
   cx<W> fin[1][FF::E0], fout[1][FF::EL];
   const int comp = (w < 3) ? w : w - 3;
+  const int c1 = (comp + 1) % 3, c2 = (comp + 2) % 3;
 #pragma unroll
   for (int m = 0; m < FF::M0; m++)
 #pragma unroll
@@ -76,16 +77,12 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
       if (w < 3 && FF::in_ok(t, m)) {
         const int n = FF::in_pos(t, m, r);
         const int o = n ^ ((n >> 4) & 15);
-        if (comp == 0) {
-          const cx<W> uy = lds[1 * N + o], uz = lds[2 * N + o], wy = lds[4 * N + o], wz = lds[5 * N + o];
-          v = {uy.x * wz.x - uz.x * wy.x, uy.y * wz.y - uz.y * wy.y};
-        } else if (comp == 1) {
-          const cx<W> ux = lds[0 * N + o], uz = lds[2 * N + o], wx = lds[3 * N + o], wz = lds[5 * N + o];
-          v = {uz.x * wx.x - ux.x * wz.x, uz.y * wx.y - ux.y * wz.y};
-        } else {
-          const cx<W> ux = lds[0 * N + o], uy = lds[1 * N + o], wx = lds[3 * N + o], wy = lds[4 * N + o];
-          v = {ux.x * wy.x - uy.x * wx.x, ux.y * wy.y - uy.y * wx.y};
-        }
+        // (u x w)_c = u_c1 w_c2 - u_c2 w_c1 with c1 = c+1, c2 = c+2 (mod 3):
+        // one code path with computed rows (three branches on c would keep
+        // registers for all of them)
+        const cx<W> u1 = lds[c1 * N + o], u2 = lds[c2 * N + o];
+        const cx<W> w1 = lds[(3 + c1) * N + o], w2 = lds[(3 + c2) * N + o];
+        v = {u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y};
       }
       fin[0][m * FF::R0 + r] = v;
     }

@@ -57,25 +57,44 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   const cx<double>* B = b + fb * kp.fs + col;
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
+  // FP64 work fields: all loads first, unconditionally from a valid element
+  // (element 0 of the column for an unused slot), so every load is in flight
+  // before the first wait; the selects come after.  FP32 work fields keep
+  // predicated loads (the 16 double values would cost the occupancy).
+  constexpr bool BATCH = sizeof(W) == 8;
+  cx<double> la[M0 * R0], lb[M0 * R0];
+  if constexpr (BATCH) {
+#pragma unroll
+    for (int m = 0; m < M0; m++)
+#pragma unroll
+      for (int r = 0; r < R0; r++) {
+        const int n = F::in_pos(t, m, r);
+        const int at = (F::in_ok(t, m) && keep_idx(n, N, K)) ? n : 0;
+        la[m * R0 + r] = A[at * zs];
+        // curl is uniform per block: the k^2 blocks skip the second field
+        lb[m * R0 + r] = curl ? B[at * zs] : cx<double>{0.0, 0.0};
+      }
+  }
 #pragma unroll
   for (int m = 0; m < M0; m++)
 #pragma unroll
     for (int r = 0; r < R0; r++) {
       const int n = F::in_pos(t, m, r);
+      const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
+      const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
       cx<double> va = {0.0, 0.0}, vb = {0.0, 0.0};
-      double fz = 0.0;
-      if (F::in_ok(t, m)) {
-        fz = kp.fkz[n];
-        if (keep_idx(n, N, K)) {
-          va = A[n * zs];
-          if (curl) vb = B[n * zs];
-          if constexpr (EMK) {
-            const bool k0 = (kx == 0) && (j == 0) && (n == 0);
-            const double e = k0 ? 0.0 : emk_of(kp, fx * fx + fy * fy + fz * fz);
-            va = {e * va.x, e * va.y};
-            vb = {e * vb.x, e * vb.y};
-          }
-        }
+      if constexpr (BATCH) {
+        va = use ? la[m * R0 + r] : cx<double>{0.0, 0.0};
+        vb = (use && curl) ? lb[m * R0 + r] : cx<double>{0.0, 0.0};
+      } else if (use) {
+        va = A[n * zs];
+        if (curl) vb = B[n * zs];
+      }
+      if constexpr (EMK) {
+        const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+        const double e = k0 ? 0.0 : emk_of(kp, fx * fx + fy * fy + fz * fz);
+        va = {e * va.x, e * va.y};
+        vb = {e * vb.x, e * vb.y};
       }
       double re, im;
       if (curl) {
@@ -258,7 +277,7 @@ template <typename W, int N>
 static hipError_t zfwd_n(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
                          hipStream_t s) {
   if constexpr (alt_T(N) > 0) {
-    if (kp.tz == alt_T(N)) return zfwd_nt<W, N, alt_T(N)>(epi, kp, work, b, out, tw, s);
+    if (kp.tze == alt_T(N)) return zfwd_nt<W, N, alt_T(N)>(epi, kp, work, b, out, tw, s);
   }
   return zfwd_nt<W, N, pencil_T(N)>(epi, kp, work, b, out, tw, s);
 }

@@ -79,8 +79,9 @@ def load():
     L.gt_fluxes.argtypes = [vp, vp]
     i32p = ctypes.POINTER(ctypes.c_int32)
     L.gt_autotune.argtypes = [vp, ip]
-    L.gt_set_tiles.argtypes = [vp, ip, ip]
-    L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_double)]
+    L.gt_set_tiles.argtypes = [vp, ip, ip, ip]
+    L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip),
+                               ctypes.POINTER(ctypes.c_double)]
     L.gt_batch_create.argtypes = [ctypes.POINTER(GtParams), ip, vp, ctypes.POINTER(vp)]
     L.gt_batch_size.argtypes = [vp, ctypes.POINTER(ip)]
     L.gt_batch_set_state.argtypes = [vp, ip, vp, vp, vp]
@@ -236,17 +237,19 @@ class Context:
         return spf
 
     def autotune(self, nrep: int = 3):
-        """Pick the fastest pencil widths; returns (tz, ty, ms per nonlinear evaluation)."""
+        """Pick the fastest pencil widths; returns (tz, tze, ty, ms per nonlinear evaluation)."""
         _check(load().gt_autotune(self.h, int(nrep)), "gt_autotune")
         return self.tiles()
 
-    def set_tiles(self, tz: int, ty: int) -> None:
-        _check(load().gt_set_tiles(self.h, int(tz), int(ty)), "gt_set_tiles")
+    def set_tiles(self, tz: int, tze: int, ty: int) -> None:
+        """Pencil widths of the z-inverse, z-forward and y passes (0: default)."""
+        _check(load().gt_set_tiles(self.h, int(tz), int(tze), int(ty)), "gt_set_tiles")
 
     def tiles(self):
-        tz, ty, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        _check(load().gt_get_tiles(self.h, ctypes.byref(tz), ctypes.byref(ty), ctypes.byref(ms)), "gt_get_tiles")
-        return tz.value, ty.value, ms.value
+        tz, tze, ty, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(load().gt_get_tiles(self.h, ctypes.byref(tz), ctypes.byref(tze), ctypes.byref(ty), ctypes.byref(ms)),
+               "gt_get_tiles")
+        return tz.value, tze.value, ty.value, ms.value
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

@@ -139,13 +139,13 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
 
 /* Launch-shape autotuner (no counterpart in the reference): times nrep
  * evaluations of the nonlinear term of the current state for each pencil
- * width of the z and y passes (8 or 16 pencils per block) and keeps the
- * fastest.  Results do not depend on the widths.  gt_set_tiles forces a
- * pair (0 = default); gt_get_tiles reports the pair and the tuned time per
- * evaluation in ms (0 before tuning). */
+ * width (8 or 16 pencils per block) of the z-inverse, z-forward and y
+ * passes and keeps the fastest.  Results do not depend on the widths.
+ * gt_set_tiles forces a triple (0 = default); gt_get_tiles reports it and
+ * the tuned time per evaluation in ms (0 before tuning). */
 int gt_autotune(gt_ctx* c, int nrep);
-int gt_set_tiles(gt_ctx* c, int tz, int ty);
-int gt_get_tiles(gt_ctx* c, int* tz, int* ty, double* ms);
+int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty);
+int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
 
 /* Batches: nmembers independent members on one grid with the same
  * parameters except omega (omegas[m], or p->omega when omegas is NULL) and

@@ -104,16 +104,17 @@ def test_tile_widths_do_not_change_results(require_gpu, n):
     results = {}
     with lib.Context(p) as g:
         g.set_state(b)
-        tz0, ty0, _ = g.tiles()
+        tz0, tze0, ty0, _ = g.tiles()
         for tz in (8, 16):
-            for ty in (8, 16):
-                g.set_tiles(tz, ty)
-                g.set_state(b)
-                g.step(2, 3)
-                results[(tz, ty)] = g.get_state()
-        tz, ty, ms = g.autotune(2)
-        assert tz in (8, 16) and ty in (8, 16) and ms > 0
-    ref = results[(tz0, ty0)]
+            for tze in (8, 16):
+                for ty in (8, 16):
+                    g.set_tiles(tz, tze, ty)
+                    g.set_state(b)
+                    g.step(2, 3)
+                    results[(tz, tze, ty)] = g.get_state()
+        tz, tze, ty, ms = g.autotune(2)
+        assert tz in (8, 16) and tze in (8, 16) and ty in (8, 16) and ms > 0
+    ref = results[(tz0, tze0, ty0)]
     for key, st in results.items():
         for x, y in zip(st, ref):
             assert np.array_equal(x, y), key
