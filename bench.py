@@ -150,7 +150,7 @@ def adjoint_bench(args):
     nb = math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in b0))
     worst, eps = 0.0, 1e-6
     t0 = time.perf_counter()
-    for k in range(16):
+    for k in range(args.fd_directions):
         d = initial.initial_state(p, seed=100 + k)
         sc = nb / math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in d))
         d = [sc * f for f in d]
@@ -168,7 +168,7 @@ def adjoint_bench(args):
         "gradient_s": float(np.median(t_fwd)) + bwd,
         "model_bytes_per_adjoint_step": 270.0 * S,
         "hbm_frac_model": 270.0 * S / (bwd / nsteps) / (HBM_PEAK_GBS * 1e9),
-        "fd_check": {"directions": 16, "eps": eps, "worst_rel_err": worst, "seconds": t_fd,
+        "fd_check": {"directions": args.fd_directions, "eps": eps, "worst_rel_err": worst, "seconds": t_fd,
                      "directions_from": "initial_state seeds 100..115 scaled to |b0|"},
         "grad_amps": [float(v) for v in ga], "shells_k": [float(v) for v in g.forcing_shells()],
         "config": {"workload": "adjoint 128^3", "nsteps": nsteps, "checkpoint_every": every, "seed": seed,
@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
+    ap.add_argument("--tiles", default="", help="tz,tze,ty: force the pencil widths instead of autotuning")
+    ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
     ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
@@ -234,7 +236,10 @@ def main():
     ctx = lib.Context(p)
     ctx.set_state(b)
     tiles = None
-    if not args.no_autotune:
+    if args.tiles:
+        ctx.set_tiles(*[int(v) for v in args.tiles.split(",")])
+        tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.tiles()))
+    elif not args.no_autotune:
         # pencil widths of the z-inverse, z-forward and y passes (bit-neutral)
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.autotune(3)))
     seed = seed0

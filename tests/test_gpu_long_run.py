@@ -1,0 +1,62 @@
+"""Long-run parity: the 256^3 rotating run (nu = 2e-4, Omega = 5, 2D2C forcing
+on (3, 4], dt = 2e-3, 2000 steps, midpoint stage 1) against the CPU oracle's
+spectra recorded every 100 steps (tests/golden/cfg2_spectra_256.npz, written
+by tests/golden/make_cfg2_spectra.py).
+
+Bar: time-averaged energy and kz = 0 spectra within 1% on every shell that
+holds energy, energy and dissipation (from glob) within 1% at every record,
+and the forcing seed after 2000 steps equal.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from gturbo3d_amd import initial, lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from make_cfg2_spectra import cfg2_params  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(HERE, "golden", "cfg2_spectra_256.npz")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="long-run fixture not generated")
+def test_long_run_spectra_match_oracle(require_gpu):
+    fx = np.load(FIXTURE)
+    steps = fx["steps"]
+    p = cfg2_params(int(fx["n"]))
+    say(f"long run {p.nx}^3, {int(steps[-1])} steps, records every {int(steps[1] - steps[0])}")
+    b = initial.initial_state(p, seed=0)
+    sps, sp2ds, globs = [], [], []
+    with lib.Context(p) as g:
+        g.set_state(b)
+        seed, done = 0, 0
+        for s in steps:
+            seed = g.step(int(s) - done, seed)
+            done = int(s)
+            sp, sp2d, _, glob = g.spectra()
+            sps.append(sp)
+            sp2ds.append(sp2d)
+            globs.append(glob)
+    assert seed == int(fx["seed"])
+    sps, sp2ds, globs = np.array(sps), np.array(sp2ds), np.array(globs)
+    for name, got, ref in (("energy", sps, fx["sp"]), ("kz=0", sp2ds, fx["sp2d"])):
+        a, r = got.mean(axis=0), ref.mean(axis=0)
+        m = r > 1e-12 * r.max()
+        err = np.abs(a[m] - r[m]) / r[m]
+        say(f"  time-averaged {name} spectrum: max relative error {err.max():.2e} over {m.sum()} shells")
+        assert err.max() < 1e-2
+    for name, sl in (("energy", slice(0, 3)), ("dissipation", slice(3, 6))):
+        a, r = globs[:, sl].sum(axis=1), fx["glob"][:, sl].sum(axis=1)
+        err = np.abs(a - r) / np.abs(r)
+        say(f"  {name} at every record: max relative error {err.max():.2e}")
+        assert err.max() < 1e-2
