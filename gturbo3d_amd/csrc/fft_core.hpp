@@ -81,6 +81,7 @@ __host__ __device__ constexpr PlanInfo plan_of(int n) {
 }
 
 __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ constexpr bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <int N>
 struct Plan {
@@ -179,8 +180,11 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 // covers one 128-byte row and every access pattern is conflict-free.
 template <int T>
 struct LayInterleaved {
+  static constexpr bool XOR_LINEAR = false;
   int p;
   __device__ __forceinline__ int off(int /*g*/, int n) const { return n * T + p; }
+  __host__ __device__ static constexpr int swz(int n) { return n; }
+  __device__ __forceinline__ int row(int /*g*/) const { return 0; }
 };
 
 // Pencils stored one after another with one pad word every 8 (stride
@@ -188,10 +192,13 @@ struct LayInterleaved {
 // spreads the stride-R writes of the early Stockham stages over the banks.
 template <int N>
 struct LayRows {
+  static constexpr bool XOR_LINEAR = false;
   int s0;     // first pencil of the thread
   int sstep;  // pencil step between the thread's G pencils
   static constexpr int STR = N + N / 8;
   __device__ __forceinline__ int off(int g, int n) const { return (s0 + g * sstep) * STR + n + (n >> 3); }
+  __host__ __device__ static constexpr int swz(int n) { return n; }
+  __device__ __forceinline__ int row(int /*g*/) const { return 0; }
 };
 
 // Rows one after another, element n of a row at n ^ ((n >> 4) & 15): the XOR
@@ -199,11 +206,21 @@ struct LayRows {
 // reading consecutive n, or 8 lanes writing n = 8j + r, hit distinct
 // 16-byte bank groups, with no pad words.  For lengths with a partial last
 // block of 8 (24, 48) the permutation stays inside that block.
+//
+// The swizzle is linear over XOR: when a and c have no bit in common,
+// swz(a | c) = swz(a) ^ swz(c).  A stage position a + c with a the thread's
+// part and c a compile-time multiple of a power of two is such a sum, so for
+// a power-of-two N the byte address of every element of a stage exchange is
+// one per-thread base XOR a constant (XOR_LINEAR), instead of a swizzle per
+// element.
 template <int N>
 struct LaySwz {
+  static constexpr bool XOR_LINEAR = (N & (N - 1)) == 0;
   int s0;     // first row of the thread
   int sstep;  // row step between the thread's G rows
-  __device__ __forceinline__ int off(int g, int n) const { return (s0 + g * sstep) * N + (n ^ ((n >> 4) & 15)); }
+  __host__ __device__ static constexpr int swz(int n) { return n ^ ((n >> 4) & 15); }
+  __device__ __forceinline__ int row(int g) const { return (s0 + g * sstep) * N; }
+  __device__ __forceinline__ int off(int g, int n) const { return row(g) + swz(n); }
 };
 
 // Synchronisation between the stages of a transform: the whole block, or
@@ -252,6 +269,7 @@ struct Fft {
     constexpr int M = cdiv(NR, P);
     constexpr bool FULL = (NR % P == 0);
     cx<T> v[G][M * R];
+    constexpr int EB = (int)sizeof(cx<T>);
     if constexpr (s == 0) {
 #pragma unroll
       for (int g = 0; g < G; g++)
@@ -263,9 +281,19 @@ struct Fft {
         const int j = t + P * m;
         if (FULL || j < NR) {
 #pragma unroll
-          for (int g = 0; g < G; g++)
+          for (int g = 0; g < G; g++) {
+            if constexpr (L::XOR_LINEAR && pow2(NR)) {
+              // j < NR and r*NR share no bit: one base, one XOR per element
+              const int b = (lay.row(g) + L::swz(j)) * EB;
+              const char* base = reinterpret_cast<const char*>(lds);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[g][m * R + r] = lds[lay.off(g, j + r * NR)];
+              for (int r = 0; r < R; r++)
+                v[g][m * R + r] = *reinterpret_cast<const cx<T>*>(base + (b ^ (L::swz(r * NR) * EB)));
+            } else {
+#pragma unroll
+              for (int r = 0; r < R; r++) v[g][m * R + r] = lds[lay.off(g, j + r * NR)];
+            }
+          }
         }
       }
     }
@@ -323,9 +351,19 @@ struct Fft {
           const int k = j % Ns;
           const int d = (j / Ns) * Ns * R + k;
 #pragma unroll
-          for (int g = 0; g < G; g++)
+          for (int g = 0; g < G; g++) {
+            if constexpr (L::XOR_LINEAR && pow2(Ns) && pow2(R)) {
+              // k < Ns, r*Ns < Ns*R and the multiple of Ns*R share no bit
+              const int b = (lay.row(g) + L::swz(d)) * EB;
+              char* base = reinterpret_cast<char*>(lds);
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lay.off(g, d + r * Ns)] = v[g][m * R + r];
+              for (int r = 0; r < R; r++)
+                *reinterpret_cast<cx<T>*>(base + (b ^ (L::swz(r * Ns) * EB))) = v[g][m * R + r];
+            } else {
+#pragma unroll
+              for (int r = 0; r < R; r++) lds[lay.off(g, d + r * Ns)] = v[g][m * R + r];
+            }
+          }
         }
       }
       stage_sync<SYNC>();

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
     for (int m = 0; m < FI::ML; m++)
       if (FI::out_ok(t, m))
 #pragma unroll
-        for (int r = 0; r < FI::RL; r++) lds[lay.off(0, FI::out_pos(t, m, r))] = out[0][m * FI::RL + r];
+        for (int r = 0; r < FI::RL; r++) lds[g * N + FI::out_pos(t, m, r)] = out[0][m * FI::RL + r];
   }
   __syncthreads();
   // (a x b)_c = a_{c+1} b_{c+2} - a_{c+2} b_{c+1}, rows in .x and .y
@@ -151,8 +151,7 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
     for (int r = 0; r < FF::R0; r++) {
       cx<double> v = {0.0, 0.0};
       if (g < 6 && FF::in_ok(t, m)) {
-        const int n = FF::in_pos(t, m, r);
-        const int o = n ^ ((n >> 4) & 15);
+        const int o = FF::in_pos(t, m, r);
         const cx<double> a1 = lds[(fa + c1) * N + o], a2 = lds[(fa + c2) * N + o];
         const cx<double> b1 = lds[(fb + c1) * N + o], b2 = lds[(fb + c2) * N + o];
         v = {a1.x * b2.x - a2.x * b1.x, a1.y * b2.y - a2.y * b1.y};
@@ -167,7 +166,7 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
     for (int m = 0; m < FF::ML; m++)
       if (FF::out_ok(t, m))
 #pragma unroll
-        for (int r = 0; r < FF::RL; r++) lds[lay.off(0, FF::out_pos(t, m, r))] = fout[0][m * FF::RL + r];
+        for (int r = 0; r < FF::RL; r++) lds[g * N + FF::out_pos(t, m, r)] = fout[0][m * FF::RL + r];
   }
   __syncthreads();
   if (g < 6) {
@@ -175,7 +174,7 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
     for (int k = t; k <= kc; k += P) {
       const int kk = (k == 0) ? 0 : N - k;
       const double h = (k == 0 || k == N / 2) ? 0.5 : 1.0;  // wgt / 2
-      const cx<double> Y = lds[lay.off(0, k)], Yc = lds[lay.off(0, kk)];
+      const cx<double> Y = lds[g * N + k], Yc = lds[g * N + kk];
       dst[r0 + k] = {h * (Y.x + Yc.x), h * (Y.y - Yc.y)};
       dst[r1 + k] = {h * (Y.y + Yc.y), h * (Yc.x - Y.x)};
     }

@@ -24,7 +24,11 @@ namespace gt {
 // wave per row the stage exchanges need only wavefront synchronisation.
 // The products are formed in registers at the forward transform's input
 // positions, groups 0-2 run the three forward transforms, and all six
-// groups unpack and store.
+// groups unpack and store.  Only the transforms' own stage exchanges use the
+// swizzled layout; the rows handed between phases are stored plainly at
+// g*N + n, since every such access is at positions consecutive across lanes
+// (conflict-free, with constant offsets instead of per-element address
+// arithmetic).
 #ifndef GT_XFUSED_WAVES_PER_EU
 #define GT_XFUSED_WAVES_PER_EU 1
 #endif
@@ -62,7 +66,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
     for (int m = 0; m < FI::ML; m++)
       if (FI::out_ok(t, m))
 #pragma unroll
-        for (int r = 0; r < FI::RL; r++) lds[lay.off(0, FI::out_pos(t, m, r))] = out[0][m * FI::RL + r];
+        for (int r = 0; r < FI::RL; r++) lds[w * N + FI::out_pos(t, m, r)] = out[0][m * FI::RL + r];
   }
   __syncthreads();
   // u x w (src/produ.f90:14-16 with a = u, b = w), component w, both rows
@@ -75,8 +79,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
     for (int r = 0; r < FF::R0; r++) {
       cx<W> v = {W(0), W(0)};
       if (w < 3 && FF::in_ok(t, m)) {
-        const int n = FF::in_pos(t, m, r);
-        const int o = n ^ ((n >> 4) & 15);
+        const int o = FF::in_pos(t, m, r);
         // (u x w)_c = u_c1 w_c2 - u_c2 w_c1 with c1 = c+1, c2 = c+2 (mod 3):
         // one code path with computed rows (three branches on c would keep
         // registers for all of them)
@@ -94,17 +97,16 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
     for (int m = 0; m < FF::ML; m++)
       if (FF::out_ok(t, m))
 #pragma unroll
-        for (int r = 0; r < FF::RL; r++) lds[lay.off(0, FF::out_pos(t, m, r))] = fout[0][m * FF::RL + r];
+        for (int r = 0; r < FF::RL; r++) lds[w * N + FF::out_pos(t, m, r)] = fout[0][m * FF::RL + r];
   }
   __syncthreads();
   // unpack the two rows of component c = w % 3 (groups c and c+3 share it)
   {
     const int c = comp;
-    const L lc{c, 0};
     cx<W>* dst = work + c * kp.fs;
     for (int k = t + P * (w / 3); k <= kc; k += 2 * P) {
       const int kk = (k == 0) ? 0 : N - k;
-      const cx<W> Y = lds[lc.off(0, k)], Yc = lds[lc.off(0, kk)];
+      const cx<W> Y = lds[c * N + k], Yc = lds[c * N + kk];
       dst[r0 + k] = {(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)};
       dst[r1 + k] = {(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)};
     }
@@ -179,14 +181,14 @@ __global__ __launch_bounds__(3 * P) void k_xr2c(KP kp, const double* __restrict_
   for (int m = 0; m < FF::ML; m++)
     if (FF::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < FF::RL; r++) lds[lay.off(0, FF::out_pos(t, m, r))] = out[0][m * FF::RL + r];
+      for (int r = 0; r < FF::RL; r++) lds[sq * N + FF::out_pos(t, m, r)] = out[0][m * FF::RL + r];
   __syncthreads();
   const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
   const long long r1 = r0 + kp.nxp;
   cx<double>* dst = work + sq * kp.fs;
   for (int k = t; k <= N / 2; k += P) {
     const int kk = (k == 0) ? 0 : N - k;
-    const cx<double> Y = lds[lay.off(0, k)], Yc = lds[lay.off(0, kk)];
+    const cx<double> Y = lds[sq * N + k], Yc = lds[sq * N + kk];
     dst[r0 + k] = {(Y.x + Yc.x) * 0.5, (Y.y - Yc.y) * 0.5};
     dst[r1 + k] = {(Y.y + Yc.y) * 0.5, (Yc.x - Y.x) * 0.5};
   }

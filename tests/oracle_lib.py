@@ -188,7 +188,6 @@ class Oracle:
     def step1(self, a, c):
         C = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in c]
         Af = [np.ascontiguousarray(f, dtype=np.complex128) for f in a]
-        lib().step1 if False else None
         lib().or_step1(self.h, *[_p(x) for x in Af + C])
         return C
 
