@@ -177,6 +177,52 @@ def adjoint_bench(args):
     g.close()
 
 
+def precision_bench(args):
+    """FP32-bulk against FP64 on the GPU: the 512^3 / 1024^3 workload (nu = 5e-5,
+    Omega = 2, 3D3C forcing (2,3] rate 0.1, dt = 1e-3) run --steps steps in each
+    precision from the same state and seed; per-field relative L2 of b and the
+    relative errors of the energy and helicity spectra and of glob.  The runs
+    use the midpoint stage 1: with dt2 = 0 (the reference's) the nonlinear term
+    advances by forward Euler and this workload diverges within 500 steps."""
+    import numpy as np
+    from gturbo3d_amd import initial, lib
+    from gturbo3d_amd.params import DT2_MIDPOINT
+    n = args.size
+    res = {}
+    for prec in ("fp64", "fp32-bulk"):
+        p = workload_params(n, prec)
+        p.dt2_mode = DT2_MIDPOINT
+        p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
+        b0 = initial.initial_state(p, seed=0)
+        g = lib.Context(p)
+        g.set_state(b0)
+        del b0
+        g.autotune(2)
+        t0 = time.perf_counter()
+        seed = g.step(args.steps, 0)
+        g.sync()
+        t = time.perf_counter() - t0
+        sp, _, hel, glob = g.spectra()
+        res[prec] = dict(b=g.get_state(), sp=sp, hel=hel, glob=glob, seed=seed, ms=1e3 * t / args.steps)
+        g.close()
+    a, r = res["fp32-bulk"], res["fp64"]
+    fields = [float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(a["b"], r["b"])]
+    m = r["sp"] > 1e-12 * r["sp"].max()
+    e_sp = float(np.max(np.abs(a["sp"][m] - r["sp"][m]) / r["sp"][m]))
+    mh = np.abs(r["hel"]) > 1e-6 * np.abs(r["hel"]).max()
+    e_hel = float(np.max(np.abs(a["hel"][mh] - r["hel"][mh]) / np.abs(r["hel"][mh])))
+    e_E = abs(a["glob"][:3].sum() - r["glob"][:3].sum()) / r["glob"][:3].sum()
+    e_D = abs(a["glob"][3:6].sum() - r["glob"][3:6].sum()) / r["glob"][3:6].sum()
+    print(json.dumps({
+        "metric": f"FP32-bulk vs FP64 after {args.steps} steps at {n}^3", "unit": "relative error",
+        "field_rel_l2": {"bx": fields[0], "by": fields[1], "bz": fields[2]},
+        "energy_spectrum_max_rel": e_sp, "helicity_spectrum_max_rel": e_hel,
+        "energy_rel": e_E, "dissipation_rel": e_D, "seeds_equal": a["seed"] == r["seed"],
+        "ms_per_step": {"fp64": r["ms"], "fp32-bulk": a["ms"]}, "speedup": r["ms"] / a["ms"],
+        "config": {"workload": f"{n}^3, nu=5e-5, Omega=2, 3D3C (2,3] rate 0.1, dt=1e-3, midpoint stage 1",
+                   "steps": args.steps}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +235,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
     ap.add_argument("--tiles", default="", help="tz,tze,ty: force the pencil widths instead of autotuning")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
-    ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint"], default="step",
+    ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint", "precision"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
@@ -221,6 +267,10 @@ def main():
 
     if args.mode == "adjoint":
         adjoint_bench(args)
+        return
+
+    if args.mode == "precision":
+        precision_bench(args)
         return
 
     n = args.size
