@@ -50,6 +50,21 @@ def cpu_baseline(p, b, seed, threads):
     return t1 - t0
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host, from /proc/cpuinfo."""
+    model, count = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("processor"):
+                    count += 1
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": count}
+
+
 def ensemble_bench(args):
     """32 members at 128^3 plus 16 at 96^3, FP64, nu = 1e-3, dt = 4e-3, 3D3C forcing (2,3]
     rate 0.1, seeds 0..47, Omega ~ U[0.5, 10] from PCG64(48); chunk-step hipGraphs."""
@@ -344,7 +359,17 @@ def main():
         tc = cpu_baseline(p, b, seed0, threads)
         cpu = {"value": n ** 3 / tc, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
                "sample": f"1 step of the same {n}^3 FP64 workload, oracle (C + OpenMP) on {threads} threads, "
-                         f"{tc:.1f} s"}
+                         f"{tc:.1f} s",
+               "fft": "the oracle's own mixed-radix Stockham (long-double twiddles); FFTW is not installed",
+               "host": host_cpu()}
+        # one core, on a bounded sample: one step of the same workload at half the size
+        h = max(n // 2, 16)
+        ph = workload_params(h, "fp64")
+        ph.famp = math.sqrt(0.1 / lib.forcing_count(ph)[0])
+        bh = initial.initial_state(ph, seed=seed0)
+        t1 = cpu_baseline(ph, bh, seed0, 1)
+        cpu["single_thread"] = {"value": h ** 3 / t1, "unit": "grid-pt·steps/s", "cores": 1,
+                                "sample": f"1 step of the {h}^3 workload on 1 thread, {t1:.1f} s"}
 
     line = {
         "metric": METRIC,

@@ -354,6 +354,9 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.alpmu = c->p.alpmu;
   kp.scale = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
   kp.visc_simple = (c->p.alpnu == 1.0 && c->p.mu == 0.0) ? 1 : 0;
+  kp.emx = kp.visc_simple ? kp.fkzn + c->nz : nullptr;
+  kp.emy = kp.visc_simple ? kp.emx + c->nxh : nullptr;
+  kp.emz = kp.visc_simple ? kp.emy + c->ny : nullptr;
   kp.resident_blocks = c->resident_blocks;
   kp.nmem = c->nmem;
   kp.bstride = 3 * c->fs;
@@ -667,6 +670,9 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
     chk(hipMemcpy(c->domega, c->omegas.data(), sizeof(double) * nm, hipMemcpyHostToDevice), "member omegas");
   std::vector<double> tab;
   for (auto* v : {&c->fkx, &c->fky, &c->fkz, &c->fkxn, &c->fkyn, &c->fkzn}) tab.insert(tab.end(), v->begin(), v->end());
+  // per-axis integrating factors exp(-0.5*dt*nu*fk^2) (simple viscosity)
+  for (auto* v : {&c->fkxs, &c->fkys, &c->fkzs})
+    for (double k2 : *v) tab.push_back(std::exp(-0.5 * c->p.dt * (c->p.nu * k2)));
   if (rc == GT_OK) chk(hipMalloc(&c->dtab, sizeof(double) * tab.size()), "tables");
   if (rc == GT_OK) chk(hipMemcpy(c->dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "tables");
   if (rc == GT_OK) chk(hipMalloc(&c->dkyext, sizeof(int16_t) * c->kyext.size()), "extents");

@@ -18,6 +18,15 @@ __device__ __forceinline__ double emk_of(const KP& kp, double fk2) {
   return e;
 }
 
+// integrating factor at mode (kx, ky, kz) with k^2 = fk2.  With simple
+// viscosity (alpnu = 1, mu = 0) exp(-0.5*dt*nu*k2) is the product of one
+// factor per axis, so three small tables replace the exponential (a few
+// roundings from it, and exactly 1 at k = 0); otherwise emk_of.
+__device__ __forceinline__ double emk_at(const KP& kp, int kx, int ky, int kz, double fk2) {
+  if (kp.emx) return (kp.emx[kx] * kp.emy[ky]) * kp.emz[kz];
+  return emk_of(kp, fk2);
+}
+
 // mode (kx, ky, kz) is zeroed by trunc: the ellipsoidal test with the sums in
 // the reference order, and k = 0 (src/trunc.f90:14-19,27); never when
 // truncate /= 1
@@ -35,6 +44,18 @@ __device__ __forceinline__ cx<W> herm_pack(cx<W> X0, cx<W> X1, bool conj_half) {
   return {X0.x - X1.y, X0.y + X1.x};
 }
 
+// Element i of a row whose start is the same for the whole wavefront, with
+// the offset formed in 32 bits (a scalar base plus an unsigned vector
+// offset, instead of a 64-bit address per lane); i * sizeof < 2^32.
+template <typename V>
+__device__ __forceinline__ V ld_row(const V* row, unsigned i) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(row) + i * (unsigned)sizeof(V));
+}
+template <typename V>
+__device__ __forceinline__ void st_row(V* row, unsigned i, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(row) + i * (unsigned)sizeof(V)) = v;
+}
+
 // Packed C2R input of one field at element n of the sequence: modes above
 // kc are zero, the imaginary parts of X[0] and X[N/2] are dropped (the C2R of
 // fft_inv, src/fft_inv.f90:12).  For FP64 rows the loads are unconditional
@@ -48,16 +69,16 @@ __device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n,
   const bool edge = (mm == 0 || mm == N / 2);
   cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
   if constexpr (sizeof(W) == 8) {
-    const int at = ok ? mm : 0;
-    X0 = r0[at];
-    X1 = r1[at];
+    const unsigned at = ok ? mm : 0;
+    X0 = ld_row(r0, at);
+    X1 = ld_row(r1, at);
     X0 = {ok ? X0.x : W(0), (ok && !edge) ? X0.y : W(0)};
     X1 = {ok ? X1.x : W(0), (ok && !edge) ? X1.y : W(0)};
   } else {
     // FP32 rows: occupancy hides the latency; predicated loads fetch less
     if (ok) {
-      X0 = r0[mm];
-      X1 = r1[mm];
+      X0 = ld_row(r0, (unsigned)mm);
+      X1 = ld_row(r1, (unsigned)mm);
     }
     if (edge) {
       X0.y = W(0);

@@ -37,6 +37,11 @@ struct KP {
   double omega, dt, dt2, nu, mu, alpnu, alpmu;
   double scale;  // 1/(NX*NY*NZ), the fft_dir normalisation
   int visc_simple;  // alpnu == 1 and mu == 0: scra = nu*k2
+  // with visc_simple, exp(-0.5*dt*nu*fk^2) per axis (nxh, ny, nz entries),
+  // whose product is the integrating factor; nullptr otherwise
+  const double* emx;
+  const double* emy;
+  const double* emz;
   int resident_blocks;  // grid size of the persistent kernels (a few blocks per CU)
   // batches: nmem independent members, member m's fields start m*bstride
   // (state arrays) or m*wstride (work array) elements further on

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
         const double fz = kp.fkz[n];
         const double k2 = fx * fx + fy * fy + fz * fz;
         const bool k0 = kxc == 0 && j == 0 && n == 0;
-        const double e = ZE ? (k0 ? 0.0 : emk_of(kp, k2)) : 1.0;
+        const double e = ZE ? (k0 ? 0.0 : emk_at(kp, kxc, j, n, k2)) : 1.0;
         if (k0) {  // invcurl leaves k = 0 as it is
           v = {wp * e * c0.x, wp * e * c0.y};
         } else {
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(T* P) void k_zfwd_adj(KP kp, const cx<double>* __re
       cx<double> v = {-ca * a.y + cb * bb.y + k2 * cc.x, ca * a.x - cb * bb.x + k2 * cc.y};
       if constexpr (POST) {
         const bool k0 = kxc == 0 && j == 0 && n == 0;
-        const double e = emk_of(kp, k2);
+        const double e = emk_at(kp, kxc, j, n, k2);
         const double ez = k0 ? 0.0 : e;
         const int o = col + n * zs;
         const cx<double> lq = lam[q * kp.fs + o], la = lam[fa * kp.fs + o], lb = lam[fb * kp.fs + o];
@@ -328,7 +328,7 @@ __device__ __forceinline__ void coriolis_adj(const KP& kp, const Mode& md, const
 __global__ void k_adj_ze(KP kp, const cx<double>* __restrict__ in, cx<double>* __restrict__ out) {
   Mode md;
   if (!mode_of(kp, (long long)blockIdx.x * blockDim.x + threadIdx.x, md)) return;
-  const double e = emk_of(kp, md.k2);
+  const double e = emk_at(kp, md.i, md.j, md.k, md.k2);
   const bool k0 = is_k0(md);
 #pragma unroll
   for (int f = 0; f < 3; f++) {
@@ -344,7 +344,7 @@ __global__ void k_adj_post(KP kp, const cx<double>* __restrict__ nu, const cx<do
                            cx<double>* __restrict__ lam_out) {
   Mode md;
   if (!mode_of(kp, (long long)blockIdx.x * blockDim.x + threadIdx.x, md)) return;
-  const double e = emk_of(kp, md.k2);
+  const double e = emk_at(kp, md.i, md.j, md.k, md.k2);
   const bool k0 = is_k0(md);
   const bool proj = kp.pruned && trunc_zero(kp, md.i, md.j, md.k);
   cx<double> co[3];

@@ -40,7 +40,10 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   using FI = Fft<W, N, P, 1, +1, L, SY>;
   using FF = Fft<W, N, P, 1, -1, L, SY>;
   __shared__ cx<W> lds[6 * N];
-  const int tid = threadIdx.x, w = tid / P, t = tid % P;
+  const int tid = threadIdx.x, t = tid % P;
+  // the group of a thread is uniform over its wavefront when P is a multiple
+  // of the wavefront size: a scalar, so row starts are scalar too
+  const int w = (P % 64 == 0) ? __builtin_amdgcn_readfirstlane(tid / P) : tid / P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   work += blockIdx.z * kp.wstride;
   const int r0 = (z * kp.ny + y0) * kp.nxp;
@@ -103,12 +106,13 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   // unpack the two rows of component c = w % 3 (groups c and c+3 share it)
   {
     const int c = comp;
-    cx<W>* dst = work + c * kp.fs;
+    cx<W>* d0 = work + c * kp.fs + r0;
+    cx<W>* d1 = d0 + kp.nxp;
     for (int k = t + P * (w / 3); k <= kc; k += 2 * P) {
       const int kk = (k == 0) ? 0 : N - k;
       const cx<W> Y = lds[c * N + k], Yc = lds[c * N + kk];
-      dst[r0 + k] = {(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)};
-      dst[r1 + k] = {(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)};
+      st_row(d0, (unsigned)k, cx<W>{(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)});
+      st_row(d1, (unsigned)k, cx<W>{(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)});
     }
   }
 }

@@ -6,7 +6,7 @@
 // pass B reads only the ky rows that pass A wrote (the columns alive under
 // the truncation) and pass D writes only those rows back.
 #include "gt_config.hpp"
-#include "gt_internal.hpp"
+#include "gt_device.hpp"
 
 namespace gt {
 
@@ -22,9 +22,12 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   work += (blockIdx.z / nfields) * kp.wstride;
   const bool on = kx <= kp.kxcut;
   const int Ky = on ? (kp.pruned ? (int)kp.kyext[kx] : (N >> 1)) : -1;
-  cx<W>* fld = work + f * kp.fs;
-  const int base = z * kp.ny * kp.nxp + kx;
   const int ys = kp.nxp;
+  // element n = t + c of the thread's pencil (c = P*m + r*N/R, a constant of
+  // the slot) sits at tile + c*ys + lane: a start uniform over the block and
+  // one 32-bit lane offset, instead of a 64-bit address per element
+  cx<W>* tile = work + f * kp.fs + (long long)z * kp.ny * kp.nxp + blockIdx.x * T;
+  const unsigned lane = p + t * ys;
   LayInterleaved<T> lay{p};
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
@@ -36,7 +39,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
       bool ld = on && F::in_ok(t, m);
       if constexpr (SIGN > 0) ld = ld && keep_idx(n, N, Ky);
       cx<W> v = {W(0), W(0)};
-      if (ld) v = fld[base + n * ys];
+      if (ld) v = ld_row(tile + (long long)(P * m + r * F::NR0) * ys, lane);
       in[0][m * R0 + r] = v;
     }
   F::run(lay, lds, tw, t, in, out);
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
       if constexpr (SIGN < 0) {
         if (!keep_idx(n, N, Ky)) continue;
       }
-      fld[base + n * ys] = out[0][m * RL + r];
+      st_row(tile + (long long)(P * m + r * F::NRL) * ys, lane, out[0][m * RL + r]);
     }
   }
 }

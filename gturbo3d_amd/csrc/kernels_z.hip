@@ -45,7 +45,6 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
   if (!__syncthreads_or(K >= 0)) return;
 
-  const int col = j * kp.nxp + kx;
   const int zs = kp.ny * kp.nxp;
   const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
   const double fy = kp.fky[j];
@@ -53,8 +52,13 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   // fields and wavenumber roles, uniform per block
   const int fa = curl ? (q == 0 ? 2 : q == 1 ? 0 : 1) : q - 3;
   const int fb = q == 0 ? 1 : q == 1 ? 2 : 0;
-  const cx<double>* A = b + fa * kp.fs + col;
-  const cx<double>* B = b + fb * kp.fs + col;
+  // element n = t + c of the thread's pencil (c = P*m + r*N/R, a constant of
+  // the slot) sits at tile + c*zs + lane: a start uniform over the block and
+  // one 32-bit lane offset, instead of a 64-bit address per element
+  const long long tile = (long long)j * kp.nxp + blockIdx.x * T;
+  const cx<double>* A = b + fa * kp.fs + tile;
+  const cx<double>* B = b + fb * kp.fs + tile;
+  const unsigned lane = p + t * zs;
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
   // FP64 work fields: all loads first, unconditionally from a valid element
@@ -69,10 +73,12 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
 #pragma unroll
       for (int r = 0; r < R0; r++) {
         const int n = F::in_pos(t, m, r);
-        const int at = (F::in_ok(t, m) && keep_idx(n, N, K)) ? n : 0;
-        la[m * R0 + r] = A[at * zs];
+        // an unused slot reads element c of the column instead
+        const unsigned o = (F::in_ok(t, m) && keep_idx(n, N, K)) ? lane : (unsigned)p;
+        const long long c = (long long)(P * m + r * F::NR0) * zs;
+        la[m * R0 + r] = ld_row(A + c, o);
         // curl is uniform per block: the k^2 blocks skip the second field
-        lb[m * R0 + r] = curl ? B[at * zs] : cx<double>{0.0, 0.0};
+        lb[m * R0 + r] = curl ? ld_row(B + c, o) : cx<double>{0.0, 0.0};
       }
   }
 #pragma unroll
@@ -87,12 +93,13 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         va = use ? la[m * R0 + r] : cx<double>{0.0, 0.0};
         vb = (use && curl) ? lb[m * R0 + r] : cx<double>{0.0, 0.0};
       } else if (use) {
-        va = A[n * zs];
-        if (curl) vb = B[n * zs];
+        const long long c = (long long)(P * m + r * F::NR0) * zs;
+        va = ld_row(A + c, lane);
+        if (curl) vb = ld_row(B + c, lane);
       }
       if constexpr (EMK) {
         const bool k0 = (kx == 0) && (j == 0) && (n == 0);
-        const double e = k0 ? 0.0 : emk_of(kp, fx * fx + fy * fy + fz * fz);
+        const double e = k0 ? 0.0 : emk_at(kp, kx < kp.nxh ? kx : kp.nxh - 1, j, n, fx * fx + fy * fy + fz * fz);
         va = {e * va.x, e * va.y};
         vb = {e * vb.x, e * vb.y};
       }
@@ -112,16 +119,29 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   LayInterleaved<T> lay{p};
   F::run(lay, lds, tw, t, in, out);
   if (K < 0) return;
-  cx<W>* dst = work + q * kp.fs + col;
+  cx<W>* dst = work + q * kp.fs + tile;
 #pragma unroll
   for (int m = 0; m < ML; m++)
     if (F::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < RL; r++) dst[F::out_pos(t, m, r) * zs] = out[0][m * RL + r];
+      for (int r = 0; r < RL; r++) st_row(dst + (long long)(P * m + r * F::NRL) * zs, lane, out[0][m * RL + r]);
 }
 
-template <typename W, int N, int T, int P, int EPI>
-__global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
+// VS: simple viscosity, the integrating factor from the per-axis tables
+// (emk_at) with no exponential in the kernel
+// Waves per SIMD the register allocation targets.  FP32 work fields: 4
+// (128 registers, a few spills), so two blocks of 8 pencils share a CU
+// instead of one.  FP64: the three transforms' outputs alone take 96
+// registers, and a bound costs more in spills than it gains.
+#ifndef GT_ZFWD_WPE64
+#define GT_ZFWD_WPE64 1
+#endif
+#ifndef GT_ZFWD_WPE32
+#define GT_ZFWD_WPE32 4
+#endif
+template <typename W, int N, int T, int P, int EPI, bool VS = false>
+__global__ __launch_bounds__(T* P)
+__attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE32))) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
                                                cx<double>* out, const cx<W>* __restrict__ tw) {
   // out may be b itself (step2 updates b in place), so neither is restrict
   using F = Fft<W, N, P, 1, -1, LayInterleaved<T>>;
@@ -139,21 +159,23 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
   if (!__syncthreads_or(K >= 0)) return;
 
-  const int col = j * kp.nxp + kx;
   const int zs = kp.ny * kp.nxp;
+  // element n = t + c of the thread's pencil at tile + c*zs + lane (as in pass A)
+  const long long tile = (long long)j * kp.nxp + blockIdx.x * T;
+  const unsigned lane = p + t * zs;
   LayInterleaved<T> lay{p};
 
   cx<W> y[3][ML * RL];
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     cx<W> in[1][M0 * R0], o[1][ML * RL];
-    const cx<W>* src = work + f * kp.fs + col;
+    const cx<W>* src = work + f * kp.fs + tile;
 #pragma unroll
     for (int m = 0; m < M0; m++)
 #pragma unroll
       for (int r = 0; r < R0; r++) {
         cx<W> v = {W(0), W(0)};
-        if (K >= 0 && F::in_ok(t, m)) v = src[F::in_pos(t, m, r) * zs];
+        if (K >= 0 && F::in_ok(t, m)) v = ld_row(src + (long long)(P * m + r * F::NR0) * zs, lane);
         in[0][m * R0 + r] = v;
       }
     F::run(lay, lds, tw, t, in, o);
@@ -166,6 +188,7 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
   const double fx = kp.fkx[kxc], fy = kp.fky[j];
   const double fxs = fx * fx, fys = fy * fy;
   const double kxn = kp.fkxn[kxc], kyn = kp.fkyn[j];
+  const double exy = VS ? kp.emx[kxc] * kp.emy[j] : 0.0;
 #pragma unroll
   for (int m = 0; m < ML; m++) {
     if (!F::out_ok(t, m)) continue;
@@ -173,10 +196,10 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
     for (int r = 0; r < RL; r++) {
       const int n = F::out_pos(t, m, r);
       if (!keep_idx(n, N, K)) continue;
-      const int q = col + n * zs;
+      const long long c = tile + (long long)(P * m + r * F::NRL) * zs;  // + lane
       const double fz = kp.fkz[n];
       const bool k0 = (kx == 0) && (j == 0) && (n == 0);
-      cx<double> a[3];
+      cx<double> a[3], bv[3];
 #pragma unroll
       for (int f = 0; f < 3; f++) a[f] = {(double)y[f][m * RL + r].x * kp.scale, (double)y[f][m * RL + r].y * kp.scale};
       const double k2 = fxs + fys + fz * fz;
@@ -192,18 +215,21 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
           a[2] = {a[2].x - sr * fz / k2, a[2].y - si * fz / k2};
         }
       } else {
-        // Coriolis with the start-of-step b (src/nlt.f90:67-81)
+        // Coriolis with the start-of-step b (src/nlt.f90:67-81); b is read
+        // once and kept for the update below
         const double of = omega * fz;
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-          const cx<double> bb = b[f * kp.fs + q];
-          a[f] = {a[f].x - of * bb.y, a[f].y + of * bb.x};
+          bv[f] = ld_row(b + f * kp.fs + c, lane);
+          a[f] = {a[f].x - of * bv[f].y, a[f].y + of * bv[f].x};
         }
       }
-      // invcurl, k = 0 left untouched (src/produ.f90:83-99)
+      // invcurl, k = 0 left untouched (src/produ.f90:83-99), with one
+      // reciprocal of k2 instead of three divisions
       cx<double> nl[3] = {a[0], a[1], a[2]};
       if (!k0) {
-        const double qx = fx / k2, qy = fy / k2, qz = fz / k2;
+        const double ik2 = 1.0 / k2;
+        const double qx = fx * ik2, qy = fy * ik2, qz = fz * ik2;
         nl[0] = {-qy * a[2].y + qz * a[1].y, qy * a[2].x - qz * a[1].x};
         nl[1] = {-qz * a[0].y + qx * a[2].y, qz * a[0].x - qx * a[2].x};
         nl[2] = {-qx * a[1].y + qy * a[0].y, qx * a[1].x - qy * a[0].x};
@@ -218,17 +244,17 @@ __global__ __launch_bounds__(T* P) void k_zfwd(KP kp, const cx<W>* __restrict__ 
       }
       if constexpr (EPI == EPI_NONLIN || EPI == EPI_INIT) {
 #pragma unroll
-        for (int f = 0; f < 3; f++) out[f * kp.fs + q] = nl[f];
+        for (int f = 0; f < 3; f++) st_row(out + f * kp.fs + c, lane, nl[f]);
       } else {
-        const double e = emk_of(kp, k2);
+        const double e = VS ? exy * kp.emz[n] : emk_of(kp, k2);
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-          const cx<double> bb = b[f * kp.fs + q];
+          const cx<double> bb = bv[f];
           cx<double> o;
           if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
           else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
           if (k0) o = {0.0, 0.0};
-          out[f * kp.fs + q] = o;
+          st_row(out + f * kp.fs + c, lane, o);
         }
       }
     }
@@ -263,9 +289,16 @@ static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b
   const cx<double>* bb = (const cx<double>*)b;
   cx<double>* o = (cx<double>*)out;
   const cx<W>* t = (const cx<W>*)tw;
+  const bool vs = kp.emx != nullptr;
   switch (epi) {
-    case EPI_STEP1: k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
-    case EPI_STEP2: k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    case EPI_STEP1:
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      break;
+    case EPI_STEP2:
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      break;
     case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
     case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
     default: return hipErrorInvalidValue;

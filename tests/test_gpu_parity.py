@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from gturbo3d_amd import initial, lib
-from gturbo3d_amd.params import DT2_MIDPOINT, DT2_REFERENCE, FORCING_2D2C, FORCING_3D3C, Params
+from gturbo3d_amd.params import (DT2_MIDPOINT, DT2_REFERENCE, FORCING_2D2C, FORCING_3D3C, PREC_FP32_BULK,
+                                 PREC_FP64, Params)
 import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
@@ -85,6 +86,22 @@ def test_one_step_large_matches_oracle(require_gpu, n):
     ob1, os1 = ol.Oracle(p).step(b, 7, 1)
     assert s == os1
     assert rel_l2(b1, ob1) < 1e-13
+
+
+def test_nonlinear_512_matches_oracle(require_gpu):
+    """The full 512^3 shape against the oracle (the x pass runs persistent
+    blocks at this size)."""
+    p = cfg(512, omega=2.0, nu=5e-5, dt=1e-3)
+    say("nonlinear 512^3")
+    b = initial.initial_state(p, seed=5)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        assert g.pruned()
+        N = g.nonlinear()
+    No, _ = ol.Oracle(p).nlt(b, b)
+    err = rel_l2(N, No)
+    say(f"  relative L2 {err:.2e}")
+    assert err < 1e-13
 
 
 def test_dense_path_without_truncation(require_gpu):
@@ -233,3 +250,55 @@ def test_state_round_trip_and_seed_only_forcing(require_gpu):
         g.set_state(b)
         s = g.step(1, 42)
     assert s == ol.rann_sequence(42, 2 * ol.Oracle(p).nf)[1]
+
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32_BULK])
+def test_quadratic_support_and_energy_at_1024(require_gpu, prec):
+    """The largest compiled size, through size-independent properties of N(b)
+    for a state b on the modes |k| <= 10: N is quadratic, so it lives on
+    |k_i| <= 20 up to transform roundoff (a wrong twiddle, index or stride in
+    any pass spreads O(1) energy outside), and it conserves the energy
+    sum fac k^2 Re(conj(b) N) = 0 (Galerkin truncation)."""
+    kmax = 10
+    p = cfg(1024, omega=2.0, nu=5e-5, dt=1e-3, precision=prec)
+    say(f"quadratic support and energy conservation 1024^3, precision {prec}")
+    b = lowmode_state(p, seed=1, kmax=kmax)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        assert g.pruned()
+        N = g.nonlinear()
+    q = 2 * kmax
+    zin = np.r_[0:q + 1, p.nz - q:p.nz]
+    yin = np.r_[0:q + 1, p.ny - q:p.ny]
+    e_in = e_out = 0.0
+    for f in N:
+        e_all = 0.0
+        # |kz| > q: contiguous slabs, no copies
+        e_out += float(np.vdot(f[q + 1:p.nz - q].ravel(), f[q + 1:p.nz - q].ravel()).real)
+        for k in zin:
+            e_out += float(np.vdot(f[k, q + 1:p.ny - q].ravel(), f[k, q + 1:p.ny - q].ravel()).real)
+            blk = f[k][yin]
+            e_out += float(np.sum(np.abs(blk[:, q + 1:]) ** 2))
+            e_all += float(np.sum(np.abs(blk[:, :q + 1]) ** 2))
+        e_in += e_all
+    ratio = e_out / e_in
+    say(f"  energy of N outside |k_i| <= {q}: {ratio:.2e} of the inside")
+    assert e_in > 0 and ratio < (1e-24 if prec == PREC_FP64 else 1e-10)
+    # energy: b vanishes outside its box, so the sum runs over that box
+    fkx, fky, fkz = initial.wavenumbers(p)
+    kb = np.r_[0:kmax + 1, p.nz - kmax:p.nz]
+    jb = np.r_[0:kmax + 1, p.ny - kmax:p.ny]
+    k2 = (fkx[None, None, :kmax + 1] ** 2 + fky[jb][None, :, None] ** 2) + fkz[kb][:, None, None] ** 2
+    fac = np.ones(kmax + 1)
+    fac[0] = 0.5
+    dE = sc = 0.0
+    for x, y in zip(b, N):
+        xs, ys = x[np.ix_(kb, jb, np.arange(kmax + 1))], y[np.ix_(kb, jb, np.arange(kmax + 1))]
+        dE += float(np.sum(fac * k2 * np.real(np.conj(xs) * ys)))
+        sc += float(np.sum(fac * k2 * np.abs(xs) * np.abs(ys)))
+    say(f"  energy transfer {abs(dE) / sc:.2e} of its scale")
+    assert sc > 0 and abs(dE) < (1e-12 if prec == PREC_FP64 else 1e-6) * sc
+    # the kx planes past the truncation ellipsoid are zeroed
+    kxcut = int(math.floor(p.rtrunc * (p.nxh - 1)))
+    assert all(not np.any(f[zin][:, :, kxcut + 1:]) for f in N)
