@@ -32,9 +32,10 @@ def workload_params(n, precision):
     return p
 
 
-def cpu_baseline(p, b, seed, threads):
-    """One oracle step of the same workload on the host cores (the oracle is
-    the CPU restatement of the reference step, OpenMP over pencils)."""
+def cpu_baseline(p, b, seed, threads, reps=1, warmup=0):
+    """Oracle steps of the same workload on the host cores (the oracle is the
+    CPU restatement of the reference step, OpenMP over pencils): `warmup`
+    untimed steps, then the median seconds of `reps` timed steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as ol
@@ -44,15 +45,42 @@ def cpu_baseline(p, b, seed, threads):
     W = [np.ones(p.shape, np.complex128) for _ in range(6)]  # pre-faulted work fields
     s = ctypes.c_int32(seed)
     ptrs = [ctypes.c_void_p(x.ctypes.data) for x in B + W]
-    t0 = time.perf_counter()
-    ol.lib().or_step(o.h, *ptrs, ctypes.byref(s), 1)
-    t1 = time.perf_counter()
-    return t1 - t0
+    for _ in range(warmup):
+        ol.lib().or_step(o.h, *ptrs, ctypes.byref(s), 1)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ol.lib().or_step(o.h, *ptrs, ctypes.byref(s), 1)
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_fft_gflops(n, threads, reps=3):
+    """Nominal rate of the oracle's 3D real-to-complex transform (fft_dir) at
+    n^3 on `threads` threads: 2.5 N log2 N flop for N = n^3 real points over
+    the median time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as ol
+    from gturbo3d_amd.params import Params
+    ol.set_threads(threads)
+    o = ol.Oracle(Params(nx=n, ny=n, nz=n))
+    rng = np.random.default_rng(1)
+    z = np.zeros((n, n, n // 2 + 1), np.complex128)
+    z.view(np.float64).reshape(n, n, n + 2)[:, :, :n] = rng.standard_normal((n, n, n))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ol.lib().or_fft_dir(o.h, ctypes.c_void_p(z.ctypes.data))
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
+    return 2.5 * n ** 3 * math.log2(n ** 3) / t / 1e9
 
 
 def host_cpu():
-    """Model name and logical CPU count of the host, from /proc/cpuinfo."""
-    model, count = "unknown", 0
+    """Model name, logical CPU and socket counts of the host (/proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, count, sockets = "unknown", 0, set()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -60,9 +88,12 @@ def host_cpu():
                     model = line.split(":", 1)[1].strip()
                 if line.startswith("processor"):
                     count += 1
+                if line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
     except OSError:
         pass
-    return {"model": model, "logical_cpus": count}
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else count
+    return {"model": model, "logical_cpus": count, "sockets": len(sockets) or None, "cpus_available": avail}
 
 
 def ensemble_bench(args):
@@ -248,7 +279,8 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
-    ap.add_argument("--tiles", default="", help="tz,tze,ty: force the pencil widths instead of autotuning")
+    ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk]: force the pencil widths (and the z chunk of passes B-D) "
+                         "instead of autotuning")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
     ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint", "precision"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
@@ -302,16 +334,23 @@ def main():
     ctx.set_state(b)
     tiles = None
     if args.tiles:
-        ctx.set_tiles(*[int(v) for v in args.tiles.split(",")])
+        tv = [int(v) for v in args.tiles.split(",")]
+        ctx.set_tiles(*tv[:3])
+        if len(tv) > 3:
+            ctx.set_zchunk(tv[3])
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.tiles()))
     elif not args.no_autotune:
-        # pencil widths of the z-inverse, z-forward and y passes (bit-neutral)
+        # pencil widths of the z-inverse, z-forward and y passes and the z
+        # chunk of the y/x/y passes (bit-neutral launch shapes)
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.autotune(3)))
+    if tiles is not None:
+        tiles["zchunk"] = ctx.zchunk()
     seed = seed0
     seed = ctx.step(args.warmup, seed)
     ctx.sync()
 
-    ctx.prof_enable(True)
+    # the timed region: the step loop alone (no per-launch events, which cost
+    # GPU time at every kernel boundary once passes B-D run in z chunks)
     torch.cuda.synchronize()
     ctx.sync()
     t0 = time.perf_counter()
@@ -319,7 +358,15 @@ def main():
     ctx.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # the same number of steps again with an event pair around every kernel
+    # launch on the context stream: per-pass times and the roofline figures
+    ctx.prof_enable(True)
+    tp0 = time.perf_counter()
+    seed = ctx.step(args.steps, seed)
+    ctx.sync()
+    tp1 = time.perf_counter()
     ms, launches, bytes_per = ctx.prof_read()
+    ctx.prof_enable(False)
     pruned = ctx.pruned()
     out = ctx.get_state()
     ctx.close()
@@ -355,11 +402,12 @@ def main():
 
     cpu = None
     if args.cpu_baseline == "auto":
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        tc = cpu_baseline(p, b, seed0, threads)
+        threads = args.cpu_threads or min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1), 16)
+        reps = 3 if n <= 512 else 1
+        tc = cpu_baseline(p, b, seed0, threads, reps=reps, warmup=1)
         cpu = {"value": n ** 3 / tc, "unit": "grid-pt·steps/s", "cores": threads, "kind": "port",
-               "sample": f"1 step of the same {n}^3 FP64 workload, oracle (C + OpenMP) on {threads} threads, "
-                         f"{tc:.1f} s",
+               "sample": f"the same {n}^3 FP64 workload, oracle (C + OpenMP) on {threads} threads: 1 warm-up "
+                         f"step, then the median of {reps} timed steps, {tc:.2f} s per step",
                "fft": "the oracle's own mixed-radix Stockham (long-double twiddles); FFTW is not installed",
                "host": host_cpu()}
         # one core, on a bounded sample: one step of the same workload at half the size
@@ -370,6 +418,9 @@ def main():
         t1 = cpu_baseline(ph, bh, seed0, 1)
         cpu["single_thread"] = {"value": h ** 3 / t1, "unit": "grid-pt·steps/s", "cores": 1,
                                 "sample": f"1 step of the {h}^3 workload on 1 thread, {t1:.1f} s"}
+        nf = min(n, 256)
+        cpu["fft_gflops_1_thread"] = round(cpu_fft_gflops(nf, 1), 3)
+        cpu["fft_gflops_sample"] = f"fft_dir at {nf}^3, 2.5 N log2 N nominal flop, median of 3"
 
     line = {
         "metric": METRIC,
@@ -394,6 +445,7 @@ def main():
         "hbm_frac_dense_model": dense_bytes * args.steps / t / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_step": alg_bytes_step,
         "alg_hbm_frac": alg_bytes_step * args.steps / t / (HBM_PEAK_GBS * 1e9),
+        "profiled_ms_per_step": 1e3 * (tp1 - tp0) / args.steps,
         "pass_ms_per_step": {lib.PASS_NAMES[q]: ms[q] / args.steps for q in range(lib.NPASS)},
         "pass_alg_gbs": {lib.PASS_NAMES[q]: (bytes_per[q] / (ms[q] / launches[q] * 1e-3) / 1e9 if launches[q] else 0.0)
                          for q in range(lib.NPASS)},

@@ -175,14 +175,26 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 // pencil the thread works on (p = pencil index within the block).
 // ---------------------------------------------------------------------------
 
-// T pencils interleaved: element n of pencil p at n*T + p.  Lanes with
-// consecutive p touch consecutive 16-byte words, so a group of 8 lanes
-// covers one 128-byte row and every access pattern is conflict-free.
-template <int T>
+// LDS banks on gfx950 (measured with the SQ_LDS_BANK_CONFLICT counter): a
+// 128-bit write is served 8 lanes at a time and is conflict-free when those
+// lanes hit distinct 16-byte slots of a 128-byte window; a 128-bit read, 16
+// lanes at a time over 256 bytes; a 64-bit write, 16 lanes over 128 bytes;
+// a 64-bit read, 32 lanes over 256 bytes.
+//
+// T pencils interleaved: element n of pencil p at n'*T + p.  With 16-byte
+// elements, or 16 pencils of 8 bytes, n' = n and every stage exchange is
+// conflict-free.  With 8 pencils of 8 bytes (SW), a 64-bit write's 16 lanes
+// span two slices t whose stage-0 outputs n = 8t + r differ only in bit 3,
+// so n' = n ^ ((n >> 3) & 1) puts them in different halves of the window
+// (a permutation of [0, N) for even N).
+template <int T, bool SW = false>
 struct LayInterleaved {
   static constexpr bool XOR_LINEAR = false;
   int p;
-  __device__ __forceinline__ int off(int /*g*/, int n) const { return n * T + p; }
+  __device__ __forceinline__ int off(int /*g*/, int n) const {
+    if constexpr (SW) return (n ^ ((n >> 3) & 1)) * T + p;
+    else return n * T + p;
+  }
   __host__ __device__ static constexpr int swz(int n) { return n; }
   __device__ __forceinline__ int row(int /*g*/) const { return 0; }
 };
@@ -201,11 +213,13 @@ struct LayRows {
   __device__ __forceinline__ int row(int /*g*/) const { return 0; }
 };
 
-// Rows one after another, element n of a row at n ^ ((n >> 4) & 15): the XOR
-// permutes each aligned block of 16 elements, so 16 consecutive lanes
-// reading consecutive n, or 8 lanes writing n = 8j + r, hit distinct
-// 16-byte bank groups, with no pad words.  For lengths with a partial last
-// block of 8 (24, 48) the permutation stays inside that block.
+// Rows one after another, element n of a row at swz(n), with no pad words.
+// For a power-of-two N, swz(n) = n ^ ((n >> 3) & 15): with the serving
+// widths above, the stage exchanges of the radix-8 stages (writes to
+// 8j + r and to 64(j/8) + j%8 + 8r, reads from j + 64r for lane j) hit
+// distinct slots in both precisions.  Other lengths keep n ^ ((n >> 4) & 15),
+// which permutes each aligned block of 16 elements and, for lengths with a
+// partial last block of 8 (24, 48), stays inside that block.
 //
 // The swizzle is linear over XOR: when a and c have no bit in common,
 // swz(a | c) = swz(a) ^ swz(c).  A stage position a + c with a the thread's
@@ -218,7 +232,9 @@ struct LaySwz {
   static constexpr bool XOR_LINEAR = (N & (N - 1)) == 0;
   int s0;     // first row of the thread
   int sstep;  // row step between the thread's G rows
-  __host__ __device__ static constexpr int swz(int n) { return n ^ ((n >> 4) & 15); }
+  __host__ __device__ static constexpr int swz(int n) {
+    return ((N & (N - 1)) == 0) ? (n ^ ((n >> 3) & 15)) : (n ^ ((n >> 4) & 15));
+  }
   __device__ __forceinline__ int row(int g) const { return (s0 + g * sstep) * N; }
   __device__ __forceinline__ int off(int g, int n) const { return row(g) + swz(n); }
 };

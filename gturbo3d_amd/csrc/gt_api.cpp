@@ -75,6 +75,7 @@ struct gt_ctx {
   long long fs = 0;
   int prec = 0;
   int resident_blocks = 1024;
+  int zchunk = 0;  // z planes per chunk of the y/x/y passes (0: all at once)
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
   int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
@@ -365,6 +366,8 @@ KP make_kp(const gt_ctx* c, bool dense) {
   kp.tz = c->tz;
   kp.ty = c->ty;
   kp.tze = c->tze;
+  kp.z0 = 0;
+  kp.zn = c->nz;
   return kp;
 }
 
@@ -423,9 +426,19 @@ struct ProfScope {
 int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out, hipStream_t st) {
   const int pr = c->prec;
   LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), st));
-  LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kp, c->work, twp(c, 1), st));
-  LAUNCH(PASS_C, launch_xfused(pr, kp, c->work, twp(c, 0), st));
-  LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), st));
+  // B, C and D in chunks of zchunk planes along z: a chunk's intermediates
+  // are read by the next pass while they are still in the device's
+  // memory-side cache (256 MB on MI355X).  The chunk size is a launch shape,
+  // picked by gt_autotune (0: all planes at once); results do not depend on it.
+  const int zc = (c->zchunk > 0 && c->zchunk < c->nz) ? c->zchunk : c->nz;
+  for (int z0 = 0; z0 < c->nz; z0 += zc) {
+    KP kc = kp;
+    kc.z0 = z0;
+    kc.zn = std::min(zc, c->nz - z0);
+    LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kc, c->work, twp(c, 1), st));
+    LAUNCH(PASS_C, launch_xfused(pr, kc, c->work, twp(c, 0), st));
+    LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), st));
+  }
   const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
   LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), st));
   return GT_OK;
@@ -535,6 +548,7 @@ void free_ctx(gt_ctx* c) {
     hipEventDestroy(r.b);
   }
   for (auto e : c->pool) hipEventDestroy(e);
+
   if (c->hforce_done) hipEventDestroy(c->hforce_done);
   hipFree(c->b);
   hipFree(c->bh);
@@ -1097,6 +1111,11 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
   by[PASS_E1] = colsA * c->nz * 3 * cw + keepE * 3 * cb + keepE * 3 * cb;
   by[PASS_E2] = by[PASS_E1];
   by[PASS_F] = (double)entries_per_step(c) * (sizeof(FEnt) + 6 * cb);
+  // passes B-D launch once per z chunk: the mean bytes of one launch
+  if (c->zchunk > 0 && c->zchunk < c->nz) {
+    const double nch = (double)((c->nz + c->zchunk - 1) / c->zchunk);
+    for (int q : {PASS_B, PASS_C, PASS_D}) by[q] /= nch;
+  }
   for (int q = 0; q < npass && q < NPASS; q++) {
     if (ms) ms[q] = c->prof_ms[q];
     if (launches) launches[q] = c->prof_n[q];
@@ -1121,6 +1140,20 @@ int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty) {
   c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
   c->tze = (tze == pencil_T(c->nz)) ? 0 : tze;
   c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
+  return GT_OK;
+}
+
+int gt_set_zchunk(gt_ctx* c, int planes) {
+  if (int rc = check_ctx(c)) return rc;
+  if (planes < 0) return fail(GT_EINVAL, "gt_set_zchunk: planes must be >= 0");
+  c->zchunk = (planes >= c->nz) ? 0 : planes;
+  return GT_OK;
+}
+
+int gt_get_zchunk(gt_ctx* c, int* planes) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!planes) return fail(GT_EINVAL, "gt_get_zchunk: null");
+  *planes = c->zchunk;
   return GT_OK;
 }
 
@@ -1193,6 +1226,37 @@ int gt_autotune(gt_ctx* c, int nrep) {
   const int iz = (c->tz ? c->tz : pencil_T(c->nz)) == cand[1], ie = (c->tze ? c->tze : pencil_T(c->nz)) == cand[1];
   const int iy = (c->ty ? c->ty : pencil_T(c->ny)) == cand[1];
   c->tune_ms = tA[iz] + tE[ie] + tY[iy] + (tall[0] - tA[0] - tE[0] - tY[0]);
+  // z chunk of passes B-D with the chosen widths: wall time of whole
+  // evaluations (the chunks add launches, so per-pass sums would flatter them)
+  if (rc == GT_OK) {
+    std::vector<int> zcand = {0};
+    for (int d : {16, 8, 4, 2})
+      if (c->nz % d == 0 && c->nz / d >= 8) zcand.push_back(c->nz / d);
+    hipEvent_t e0 = pool_get(c), e1 = pool_get(c);
+    if (!e0 || !e1) rc = fail(GT_EHIP, "autotune events");
+    const KP kp = make_kp(c, false);
+    // two interleaved rounds over the candidates, the best time of each kept
+    std::vector<double> tz(zcand.size(), 1e30);
+    for (int round = 0; round < 2; round++)
+      for (size_t i = 0; i < zcand.size() && rc == GT_OK; i++) {
+        c->zchunk = zcand[i];
+        rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
+        if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune record");
+        for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);
+        float t = 0.f;
+        if (rc == GT_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+          rc = fail(GT_EHIP, "autotune timing");
+        if (rc == GT_OK) tz[i] = std::min(tz[i], (double)t / nrep);
+      }
+    size_t bi = 0;
+    for (size_t i = 1; i < zcand.size(); i++)
+      if (tz[i] < tz[bi]) bi = i;
+    c->zchunk = zcand[bi];
+    if (rc == GT_OK) c->tune_ms = tz[bi];
+    if (e0) c->pool.push_back(e0);
+    if (e1) c->pool.push_back(e1);
+  }
   if (rc != GT_OK) return rc;
   return set_mode(c, c->pruned);  // bh vanishes outside the kept extents again
 }

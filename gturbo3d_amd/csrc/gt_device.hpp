@@ -3,7 +3,18 @@
 #pragma once
 #include "gt_internal.hpp"
 
+#include <type_traits>
+
 namespace gt {
+
+// f(std::integral_constant<int, I>) for I = B .. E-1, unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 // integrating factor exp(-0.5*dt*(nu*k2**alpnu + mu*k2**alpmu)), 1 at k = 0
 // (src/defk.f90:57-68), evaluated on the fly instead of read from a table
@@ -86,6 +97,50 @@ __device__ __forceinline__ cx<W> c2r_in(const cx<W>* r0, const cx<W>* r1, int n,
     }
   }
   return herm_pack(X0, X1, n > N / 2);
+}
+
+// c2r_in at position n = t + C of a thread's slot, for a compile-time
+// offset C and 0 <= t < P: when the slot lies wholly below N/2 or wholly
+// above it, the half (and so the index, the conjugation and the edge test)
+// is known at compile time.  Same values as c2r_in.
+template <typename W, int N, int P, int C>
+__device__ __forceinline__ cx<W> c2r_in_at(const cx<W>* r0, const cx<W>* r1, int t, int kc) {
+  if constexpr (C + P - 1 < N / 2) {
+    const int mm = t + C;
+    const bool ok = mm <= kc;
+    cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
+    if constexpr (sizeof(W) == 8) {
+      const unsigned at = ok ? mm : 0;
+      X0 = ld_row(r0, at);
+      X1 = ld_row(r1, at);
+      X0 = {ok ? X0.x : W(0), ok ? X0.y : W(0)};
+      X1 = {ok ? X1.x : W(0), ok ? X1.y : W(0)};
+    } else if (ok) {
+      X0 = ld_row(r0, (unsigned)mm);
+      X1 = ld_row(r1, (unsigned)mm);
+    }
+    if constexpr (C == 0) {
+      if (t == 0) X0.y = X1.y = W(0);  // X[0]: imaginary parts dropped
+    }
+    return herm_pack(X0, X1, false);
+  } else if constexpr (C > N / 2) {
+    const int mm = N - C - t;  // in (0, N/2)
+    const bool ok = mm <= kc;
+    cx<W> X0 = {W(0), W(0)}, X1 = {W(0), W(0)};
+    if constexpr (sizeof(W) == 8) {
+      const unsigned at = ok ? mm : 0;
+      X0 = ld_row(r0, at);
+      X1 = ld_row(r1, at);
+      X0 = {ok ? X0.x : W(0), ok ? X0.y : W(0)};
+      X1 = {ok ? X1.x : W(0), ok ? X1.y : W(0)};
+    } else if (ok) {
+      X0 = ld_row(r0, (unsigned)mm);
+      X1 = ld_row(r1, (unsigned)mm);
+    }
+    return herm_pack(X0, X1, true);
+  } else {
+    return c2r_in<W, N>(r0, r1, t + C, kc);
+  }
 }
 
 }  // namespace gt

@@ -50,6 +50,8 @@ struct KP {
   const double* omega_m;  // per-member omega, or nullptr for omega
   int tz, ty;  // pencils per block of the z-inverse and y passes (0: the default)
   int tze;     // pencils per block of the z-forward pass (0: the default)
+  // planes z0 .. z0+zn-1 of the y and fused x passes (all of them by default)
+  int z0, zn;
 };
 
 __device__ __forceinline__ bool keep_idx(int n, int N, int K) {

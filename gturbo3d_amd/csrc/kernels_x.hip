@@ -29,10 +29,19 @@ namespace gt {
 // g*N + n, since every such access is at positions consecutive across lanes
 // (conflict-free, with constant offsets instead of per-element address
 // arithmetic).
+//
+// PZ (pruned, kxcut < 3N/8): a thread slot whose positions all lie in
+// [3N/8, 5N/8) holds only modes above kxcut, so its inverse input is zero
+// (no loads) and its forward output is never unpacked (not stored, and the
+// butterfly outputs feeding it are dropped).
 #ifndef GT_XFUSED_WAVES_PER_EU
 #define GT_XFUSED_WAVES_PER_EU 1
 #endif
-template <typename W, int N, int P>
+template <int N, int P, int C>
+__host__ __device__ constexpr bool mid_slot() {
+  return 8 * C >= 3 * N && 8 * (C + P) <= 5 * N;
+}
+template <typename W, int N, int P, bool PZ>
 __global__ __launch_bounds__(6 * P) __attribute__((amdgpu_waves_per_eu(GT_XFUSED_WAVES_PER_EU)))
 void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   using L = LaySwz<N>;
@@ -44,7 +53,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   // the group of a thread is uniform over its wavefront when P is a multiple
   // of the wavefront size: a scalar, so row starts are scalar too
   const int w = (P % 64 == 0) ? __builtin_amdgcn_readfirstlane(tid / P) : tid / P;
-  const int y0 = 2 * blockIdx.x, z = blockIdx.y;
+  const int y0 = 2 * blockIdx.x, z = kp.z0 + blockIdx.y;
   work += blockIdx.z * kp.wstride;
   const int r0 = (z * kp.ny + y0) * kp.nxp;
   const int r1 = r0 + kp.nxp;
@@ -55,14 +64,15 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   {
     cx<W> in[1][FI::E0], out[1][FI::EL];
     const cx<W>* src = work + w * kp.fs;
-#pragma unroll
-    for (int m = 0; m < FI::M0; m++)
-#pragma unroll
-      for (int r = 0; r < FI::R0; r++) {
-        cx<W> v = {W(0), W(0)};
-        if (FI::in_ok(t, m)) v = c2r_in<W, N>(src + r0, src + r1, FI::in_pos(t, m, r), kc);
-        in[0][m * FI::R0 + r] = v;
+    static_for<0, FI::M0 * FI::R0>([&](auto qc) {
+      constexpr int q = decltype(qc)::value, m = q / FI::R0, r = q % FI::R0;
+      constexpr int C = P * m + r * FI::NR0;  // in_pos(t, m, r) = t + C
+      cx<W> v = {W(0), W(0)};
+      if constexpr (!(PZ && mid_slot<N, P, C>())) {
+        if (FI::in_ok(t, m)) v = c2r_in_at<W, N, P, C>(src + r0, src + r1, t, kc);
       }
+      in[0][q] = v;
+    });
     FI::run(lay, lds, tw, t, in, out);
     stage_sync<SY>();
 #pragma unroll
@@ -96,11 +106,12 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   if (SY == SYNC_BLOCK || w < 3) {
     FF::run(lay, lds, tw, t, fin, fout);
     stage_sync<SY>();
-#pragma unroll
-    for (int m = 0; m < FF::ML; m++)
-      if (FF::out_ok(t, m))
-#pragma unroll
-        for (int r = 0; r < FF::RL; r++) lds[w * N + FF::out_pos(t, m, r)] = fout[0][m * FF::RL + r];
+    static_for<0, FF::ML * FF::RL>([&](auto qc) {
+      constexpr int q = decltype(qc)::value, m = q / FF::RL, r = q % FF::RL;
+      if constexpr (!(PZ && mid_slot<N, P, P * m + r * FF::NRL>())) {
+        if (FF::out_ok(t, m)) lds[w * N + FF::out_pos(t, m, r)] = fout[0][q];
+      }
+    });
   }
   __syncthreads();
   // unpack the two rows of component c = w % 3 (groups c and c+3 share it)
@@ -232,8 +243,11 @@ __global__ void k_forcing_seq(cx<double>* b, long long fs, const FEnt* __restric
 template <typename W, int N>
 static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = row_P(N);
-  dim3 grid(kp.ny / 2, kp.nz, kp.nmem);
-  k_xfused<W, N, P><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  dim3 grid(kp.ny / 2, kp.zn, kp.nmem);
+  if (8 * kp.kxcut < 3 * N)
+    k_xfused<W, N, P, true><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+  else
+    k_xfused<W, N, P, false><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }
 

@@ -12,12 +12,13 @@ namespace gt {
 
 template <typename W, int N, int T, int P, int SIGN>
 __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* __restrict__ tw, int nfields) {
-  using F = Fft<W, N, P, 1, SIGN, LayInterleaved<T>>;
+  using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
+  using F = Fft<W, N, P, 1, SIGN, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
-  const int z = blockIdx.y;
+  const int z = kp.z0 + blockIdx.y;
   const int f = blockIdx.z % nfields;
   work += (blockIdx.z / nfields) * kp.wstride;
   const bool on = kx <= kp.kxcut;
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   // one 32-bit lane offset, instead of a 64-bit address per element
   cx<W>* tile = work + f * kp.fs + (long long)z * kp.ny * kp.nxp + blockIdx.x * T;
   const unsigned lane = p + t * ys;
-  LayInterleaved<T> lay{p};
+  LI lay{p};
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
 #pragma unroll
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
 template <typename W, int N, int T>
 static hipError_t yfft_nt(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.nz, nfields * kp.nmem);
+  dim3 grid((kp.kxcut + T) / T, kp.zn, nfields * kp.nmem);
   if (sign > 0)
     k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
   else

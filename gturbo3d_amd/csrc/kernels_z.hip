@@ -21,7 +21,14 @@ namespace gt {
 
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 
-// Pass A: one output component q = blockIdx.z per block (0..2: u = curl b,
+#ifndef GT_ZINV_XCD
+#define GT_ZINV_XCD 1
+#endif
+#ifndef GT_ZFWD_PAIR
+#define GT_ZFWD_PAIR 1
+#endif
+
+// Pass A: one output component q per block (0..2: u = curl b,
 // 3..5: w = k^2 b), so a block holds only the one or two fields of b it
 // needs in registers.
 //   u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
@@ -31,14 +38,29 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 template <typename W, int N, int T, int P, bool EMK = false>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
                                                const cx<W>* __restrict__ tw, int nout) {
-  using F = Fft<W, N, P, 1, +1, LayInterleaved<T>>;
+  using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
+  using F = Fft<W, N, P, 1, +1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
-  const int kx = blockIdx.x * T + p;
-  const int j = blockIdx.y;
-  const int q = blockIdx.z % nout;
-  const int mem = blockIdx.z / nout;
+  // 1-D grid over (column, q).  Workgroups go to the 8 XCDs in turn, so the
+  // nout blocks of one column are given the consecutive slots of one XCD
+  // (ids i, i+8, i+16, ...): they read the same fields of b close in time
+  // and all but the first find them in that XCD's L2.  A tail of fewer than
+  // 8*nout blocks keeps the plain order.
+  const int L = blockIdx.x, grp = 8 * nout, bulk = (gridDim.x / grp) * grp;
+  int col, q;
+  if (GT_ZINV_XCD && L < bulk) {
+    const int s = L >> 3;
+    q = s % nout;
+    col = 8 * (s / nout) + (L & 7);
+  } else {
+    q = L % nout;
+    col = L / nout;
+  }
+  const int ntx = (kp.kxcut + T) / T;
+  const int tx = col % ntx, j = (col / ntx) % kp.ny, mem = col / (ntx * kp.ny);
+  const int kx = tx * T + p;
   b += mem * kp.bstride;
   work += mem * kp.wstride;
   int K = -1;
@@ -55,7 +77,7 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   // element n = t + c of the thread's pencil (c = P*m + r*N/R, a constant of
   // the slot) sits at tile + c*zs + lane: a start uniform over the block and
   // one 32-bit lane offset, instead of a 64-bit address per element
-  const long long tile = (long long)j * kp.nxp + blockIdx.x * T;
+  const long long tile = (long long)j * kp.nxp + tx * T;
   const cx<double>* A = b + fa * kp.fs + tile;
   const cx<double>* B = b + fb * kp.fs + tile;
   const unsigned lane = p + t * zs;
@@ -116,7 +138,7 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
       }
       in[0][m * R0 + r] = {(W)re, (W)im};
     }
-  LayInterleaved<T> lay{p};
+  LI lay{p};
   F::run(lay, lds, tw, t, in, out);
   if (K < 0) return;
   cx<W>* dst = work + q * kp.fs + tile;
@@ -144,13 +166,26 @@ __global__ __launch_bounds__(T* P)
 __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE32))) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
                                                cx<double>* out, const cx<W>* __restrict__ tw) {
   // out may be b itself (step2 updates b in place), so neither is restrict
-  using F = Fft<W, N, P, 1, -1, LayInterleaved<T>>;
+  using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
+  using F = Fft<W, N, P, 1, -1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
   __shared__ cx<W> lds[N * T];
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
-  const int kx = blockIdx.x * T + p;
-  const int j = blockIdx.y;
-  const int mem = blockIdx.z;
+  // 1-D grid over columns (kx tile, ky, member).  When a tile row is shorter
+  // than a 128-byte line (FP32 work fields, 8 pencils), PF adjacent tiles
+  // share each line: they are given consecutive slots of one XCD (workgroups
+  // go to the 8 XCDs in turn), so the line is fetched once into that XCD's
+  // L2.  A tail of fewer than 8*PF blocks keeps the plain order.
+  constexpr int PF = (T * (int)sizeof(cx<W>) >= 128) ? 1 : 128 / (T * (int)sizeof(cx<W>));
+  const int L = blockIdx.x, grp = 8 * PF, bulk = (gridDim.x / grp) * grp;
+  int col = L;
+  if (GT_ZFWD_PAIR && PF > 1 && L < bulk) {
+    const int s = L >> 3;
+    col = PF * (8 * (s / PF) + (L & 7)) + s % PF;
+  }
+  const int ntx = (kp.kxcut + T) / T;
+  const int tx = col % ntx, j = (col / ntx) % kp.ny, mem = col / (ntx * kp.ny);
+  const int kx = tx * T + p;
   work += mem * kp.wstride;
   if (b) b += mem * kp.bstride;
   out += mem * kp.bstride;
@@ -161,9 +196,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 
   const int zs = kp.ny * kp.nxp;
   // element n = t + c of the thread's pencil at tile + c*zs + lane (as in pass A)
-  const long long tile = (long long)j * kp.nxp + blockIdx.x * T;
+  const long long tile = (long long)j * kp.nxp + tx * T;
   const unsigned lane = p + t * zs;
-  LayInterleaved<T> lay{p};
+  LI lay{p};
 
   cx<W> y[3][ML * RL];
 #pragma unroll
@@ -267,7 +302,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 template <typename W, int N, int T>
 static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.ny, nout * kp.nmem);
+  const int grid = (kp.kxcut + T) / T * kp.ny * nout * kp.nmem;
   k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
   return hipGetLastError();
 }
@@ -284,7 +319,7 @@ template <typename W, int N, int T>
 static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b, void* out, const void* tw,
                           hipStream_t s) {
   constexpr int P = pencil_P(N);
-  dim3 grid((kp.kxcut + T) / T, kp.ny, kp.nmem);
+  const int grid = (kp.kxcut + T) / T * kp.ny * kp.nmem;
   const cx<W>* w = (const cx<W>*)work;
   const cx<double>* bb = (const cx<double>*)b;
   cx<double>* o = (cx<double>*)out;
@@ -335,7 +370,7 @@ hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* 
 #define GT_CASE(N)                                                                                     \
   case N: {                                                                                            \
     constexpr int T = pencil_T(N), P = pencil_P(N);                                                    \
-    dim3 grid((kp.kxcut + T) / T, kp.ny, 6);                                                           \
+    const int grid = (kp.kxcut + T) / T * kp.ny * 6;                                                   \
     k_zinv<double, N, T, P, true><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<double>*)work,  \
                                                           (const cx<double>*)tw, 6);                   \
     return hipGetLastError();                                                                          \

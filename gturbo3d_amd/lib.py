@@ -27,7 +27,7 @@ EXPORTS = [
     "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
-    "gt_get_tiles", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
+    "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
 ]
 
 
@@ -80,6 +80,8 @@ def load():
     i32p = ctypes.POINTER(ctypes.c_int32)
     L.gt_autotune.argtypes = [vp, ip]
     L.gt_set_tiles.argtypes = [vp, ip, ip, ip]
+    L.gt_set_zchunk.argtypes = [vp, ip]
+    L.gt_get_zchunk.argtypes = [vp, ctypes.POINTER(ip)]
     L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip),
                                ctypes.POINTER(ctypes.c_double)]
     L.gt_batch_create.argtypes = [ctypes.POINTER(GtParams), ip, vp, ctypes.POINTER(vp)]
@@ -237,13 +239,22 @@ class Context:
         return spf
 
     def autotune(self, nrep: int = 3):
-        """Pick the fastest pencil widths; returns (tz, tze, ty, ms per nonlinear evaluation)."""
+        """Pick the fastest pencil widths and z chunk; returns (tz, tze, ty, ms per nonlinear evaluation)."""
         _check(load().gt_autotune(self.h, int(nrep)), "gt_autotune")
         return self.tiles()
 
     def set_tiles(self, tz: int, tze: int, ty: int) -> None:
         """Pencil widths of the z-inverse, z-forward and y passes (0: default)."""
         _check(load().gt_set_tiles(self.h, int(tz), int(tze), int(ty)), "gt_set_tiles")
+
+    def set_zchunk(self, planes: int) -> None:
+        """z planes per chunk of the y-inverse, x and y-forward passes (0: all at once)."""
+        _check(load().gt_set_zchunk(self.h, int(planes)), "gt_set_zchunk")
+
+    def zchunk(self) -> int:
+        v = ctypes.c_int()
+        _check(load().gt_get_zchunk(self.h, ctypes.byref(v)), "gt_get_zchunk")
+        return v.value
 
     def tiles(self):
         tz, tze, ty, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

@@ -133,19 +133,28 @@ int gt_pruned(gt_ctx* c, int* pruned);
  * 6 F (forcing).  gt_prof_read waits for the stream and returns the summed
  * milliseconds and launch counts since enable; bytes[] is the algorithmic
  * HBM traffic of one launch of each pass (every element the pass must
- * read or write, once). */
+ * read or write, once; for passes B-D run in z chunks, the mean over the
+ * chunks). */
 int gt_prof_enable(gt_ctx* c, int on);
 int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int npass);
 
 /* Launch-shape autotuner (no counterpart in the reference): times nrep
  * evaluations of the nonlinear term of the current state for each pencil
  * width (8 or 16 pencils per block) of the z-inverse, z-forward and y
- * passes and keeps the fastest.  Results do not depend on the widths.
+ * passes and keeps the fastest, then the z chunk below.  Results do not
+ * depend on the widths.
  * gt_set_tiles forces a triple (0 = default); gt_get_tiles reports it and
  * the tuned time per evaluation in ms (0 before tuning). */
 int gt_autotune(gt_ctx* c, int nrep);
 int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty);
 int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
+/* The y-inverse, x and y-forward passes run in chunks of `planes` z planes
+ * (0: all at once), so that each chunk's intermediates are read back while
+ * they sit in the device's memory-side cache.  gt_autotune picks it (after
+ * the widths, from whole-evaluation times; gt_get_tiles then reports that
+ * time).  Results do not depend on it.  No counterpart in the reference. */
+int gt_set_zchunk(gt_ctx* c, int planes);
+int gt_get_zchunk(gt_ctx* c, int* planes);
 
 /* Batches: nmembers independent members on one grid with the same
  * parameters except omega (omegas[m], or p->omega when omegas is NULL) and
