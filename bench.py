@@ -394,11 +394,19 @@ def main():
         "avg_launch_ms": round(avg_ms, 4),
         "alg_bytes_per_launch": bytes_per[dom],
     }
+    # HBM bytes per launch from the committed rocprofv3 counter runs (FETCH_SIZE
+    # and WRITE_SIZE, calibrated on a device copy); passes B-D launch once per
+    # z chunk, so their figure scales with the chunk size of this run
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.precision}_{n}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
-        roofline["traffic"] = tr.get(lib.PASS_NAMES[dom])
+        name = lib.PASS_NAMES[dom]
+        tb = tr.get("per_launch", {}).get(name)
+        if tb is not None and name in ("B_yinv", "C_x_cross", "D_yfwd"):
+            zc_run = (tiles or {}).get("zchunk") or n
+            tb *= zc_run / (tr.get("zchunk") or n)
+        roofline["traffic"] = tb
 
     cpu = None
     if args.cpu_baseline == "auto":

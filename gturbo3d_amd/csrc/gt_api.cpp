@@ -1104,7 +1104,7 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
   (void)elemsB;
   const double full = (double)(kxc + 1) * c->ny * c->nz;
   double by[NPASS] = {0};
-  by[PASS_A] = keepE * 9 * cb + colsA * c->nz * 6 * cw;  // each of the 6 outputs reads 2 (curl) or 1 (k^2) fields of b
+  by[PASS_A] = keepE * 3 * cb + colsA * c->nz * 6 * cw;  // b once per field (the six outputs re-read it, mostly from L2)
   by[PASS_B] = colsY * c->nz * 6 * cw + full * 6 * cw;
   by[PASS_C] = full * 6 * cw + full * 3 * cw;
   by[PASS_D] = full * 3 * cw + colsY * c->nz * 3 * cw;
