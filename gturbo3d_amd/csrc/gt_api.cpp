@@ -1176,19 +1176,22 @@ int gt_autotune(gt_ctx* c, int nrep) {
   if (int rc = check_ctx(c)) return rc;
   if (nrep < 1) return fail(GT_EINVAL, "gt_autotune: nrep must be >= 1");
   if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
-  // only lengths with a second width for both axes have a choice
-  std::vector<int> cand = {8, 16};
+  const std::vector<int> cand = {8, 16};
+  const int NC = (int)cand.size();
   const bool prof = c->prof;
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<gt_ctx::Rec> saved;
   saved.swap(c->recs);
-  double tA[2] = {1e30, 1e30}, tE[2] = {1e30, 1e30}, tY[2] = {1e30, 1e30}, tall[2] = {0, 0};
+  std::vector<double> tA(NC, 1e30), tE(NC, 1e30), tY(NC, 1e30);
   int rc = GT_OK;
   auto ok = [](int n, int t) { return t == pencil_T(n) || (alt_T(n) > 0 && t == alt_T(n)); };
-  for (int ci = 0; ci < 2 && rc == GT_OK; ci++) {
+  auto enc = [](int n, int t) { return t == pencil_T(n) ? 0 : t; };
+  for (int ci = 0; ci < NC && rc == GT_OK; ci++) {
     const int w = cand[ci];
-    c->tz = c->tze = ok(c->nz, w) ? (w == pencil_T(c->nz) ? 0 : w) : 0;
-    c->ty = ok(c->ny, w) ? (w == pencil_T(c->ny) ? 0 : w) : 0;
+    if (!ok(c->nz, w) && !ok(c->ny, w)) continue;  // no pass has this width
+    c->tz = ok(c->nz, w) ? enc(c->nz, w) : 0;
+    c->tze = ok(c->nz, w) ? enc(c->nz, w) : 0;
+    c->ty = ok(c->ny, w) ? enc(c->ny, w) : 0;
     const KP kp = make_kp(c, false);
     c->prof = false;
     rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
@@ -1205,27 +1208,21 @@ int gt_autotune(gt_ctx* c, int nrep) {
       c->pool.push_back(q.b);
     }
     c->recs.clear();
-    if (ok(c->nz, w)) {
-      tA[ci] = ms[PASS_A] / nrep;
-      tE[ci] = ms[PASS_E1] / nrep;
-    }
+    if (ok(c->nz, w)) tA[ci] = ms[PASS_A] / nrep;
+    if (ok(c->nz, w)) tE[ci] = ms[PASS_E1] / nrep;
     if (ok(c->ny, w)) tY[ci] = (ms[PASS_B] + ms[PASS_D]) / nrep;
-    for (int q = 0; q < NPASS; q++) tall[ci] += ms[q] / nrep;
   }
   c->recs.swap(saved);
   c->prof = prof;
-  auto pick = [&](const double* t, int n) {
-    const int w = (t[1] < t[0]) ? cand[1] : cand[0];
-    return (w == pencil_T(n) || !(alt_T(n) > 0 && w == alt_T(n))) ? 0 : w;
+  auto pick = [&](const std::vector<double>& t, int n) {
+    int bi = -1;
+    for (int i = 0; i < NC; i++)
+      if (t[i] < 1e30 && (bi < 0 || t[i] < t[bi])) bi = i;
+    return bi < 0 ? 0 : enc(n, cand[bi]);
   };
   c->tz = pick(tA, c->nz);
   c->tze = pick(tE, c->nz);
   c->ty = pick(tY, c->ny);
-  // the tuned time per evaluation: the chosen passes plus the others (pass
-  // C, forcing) as timed with width 8, which every length has
-  const int iz = (c->tz ? c->tz : pencil_T(c->nz)) == cand[1], ie = (c->tze ? c->tze : pencil_T(c->nz)) == cand[1];
-  const int iy = (c->ty ? c->ty : pencil_T(c->ny)) == cand[1];
-  c->tune_ms = tA[iz] + tE[ie] + tY[iy] + (tall[0] - tA[0] - tE[0] - tY[0]);
   // z chunk of passes B-D with the chosen widths: wall time of whole
   // evaluations (the chunks add launches, so per-pass sums would flatter them)
   if (rc == GT_OK) {
@@ -1253,7 +1250,7 @@ int gt_autotune(gt_ctx* c, int nrep) {
     for (size_t i = 1; i < zcand.size(); i++)
       if (tz[i] < tz[bi]) bi = i;
     c->zchunk = zcand[bi];
-    if (rc == GT_OK) c->tune_ms = tz[bi];
+    if (rc == GT_OK) c->tune_ms = tz[bi];  // the tuned time per evaluation
     if (e0) c->pool.push_back(e0);
     if (e1) c->pool.push_back(e1);
   }

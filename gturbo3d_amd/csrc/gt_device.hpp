@@ -38,6 +38,11 @@ __device__ __forceinline__ double emk_at(const KP& kp, int kx, int ky, int kz, d
   return emk_of(kp, fk2);
 }
 
+// Whether any thread of the wavefront has alive = true.  In the pencil
+// kernels every wavefront holds all T pencils of its block (pencil = lane
+// mod T), so this is the block's answer too, with no LDS word and no wait.
+__device__ __forceinline__ bool wave_any(bool alive) { return __any(alive ? 1 : 0) != 0; }
+
 // mode (kx, ky, kz) is zeroed by trunc: the ellipsoidal test with the sums in
 // the reference order, and k = 0 (src/trunc.f90:14-19,27); never when
 // truncate /= 1

@@ -43,7 +43,8 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
   const int q = blockIdx.z;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
-  if (!__syncthreads_or(K >= 0)) return;
+  static_assert(64 % T == 0, "every wavefront holds all T pencils");
+  if (!wave_any(K >= 0)) return;  // uniform over the block
   const int col = j * kp.nxp + kx;
   const int zs = kp.ny * kp.nxp;
   const int kxc = kx < kp.nxh ? kx : kp.nxh - 1;
@@ -200,7 +201,8 @@ __global__ __launch_bounds__(T* P) void k_zfwd_adj(KP kp, const cx<double>* __re
   const int q = blockIdx.z;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
-  if (!__syncthreads_or(K >= 0)) return;
+  static_assert(64 % T == 0, "every wavefront holds all T pencils");
+  if (!wave_any(K >= 0)) return;  // uniform over the block
   const int col = j * kp.nxp + kx;
   const int zs = kp.ny * kp.nxp;
   LayInterleaved<T> lay{p};

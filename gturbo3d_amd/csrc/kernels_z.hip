@@ -46,14 +46,17 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   // 1-D grid over (column, q).  Workgroups go to the 8 XCDs in turn, so the
   // nout blocks of one column are given the consecutive slots of one XCD
   // (ids i, i+8, i+16, ...): they read the same fields of b close in time
-  // and all but the first find them in that XCD's L2.  A tail of fewer than
-  // 8*nout blocks keeps the plain order.
-  const int L = blockIdx.x, grp = 8 * nout, bulk = (gridDim.x / grp) * grp;
+  // and all but the first find them in that XCD's L2.  When a tile row is
+  // shorter than a 128-byte line, the PF adjacent tiles that share the line
+  // follow on the same XCD too.  A tail of fewer than 8*nout*PF blocks keeps
+  // the plain order.
+  constexpr int PF = (T * (int)sizeof(cx<W>) >= 128) ? 1 : 128 / (T * (int)sizeof(cx<W>));
+  const int L = blockIdx.x, grp = 8 * nout * PF, bulk = (gridDim.x / grp) * grp;
   int col, q;
   if (GT_ZINV_XCD && L < bulk) {
-    const int s = L >> 3;
+    const int s = L >> 3, u = s / nout;
     q = s % nout;
-    col = 8 * (s / nout) + (L & 7);
+    col = PF * (8 * (u / PF) + (L & 7)) + u % PF;
   } else {
     q = L % nout;
     col = L / nout;
@@ -65,7 +68,8 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   work += mem * kp.wstride;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
-  if (!__syncthreads_or(K >= 0)) return;
+  static_assert(64 % T == 0, "every wavefront holds all T pencils");
+  if (!wave_any(K >= 0)) return;  // uniform over the block
 
   const int zs = kp.ny * kp.nxp;
   const double fx = kp.fkx[kx < kp.nxh ? kx : kp.nxh - 1];
@@ -192,7 +196,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   const double omega = kp.omega_m ? kp.omega_m[mem] : kp.omega;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
-  if (!__syncthreads_or(K >= 0)) return;
+  static_assert(64 % T == 0, "every wavefront holds all T pencils");
+  if (!wave_any(K >= 0)) return;  // uniform over the block
 
   const int zs = kp.ny * kp.nxp;
   // element n = t + c of the thread's pencil at tile + c*zs + lane (as in pass A)

@@ -98,6 +98,11 @@ def test_nonlinear_512_matches_oracle(require_gpu):
         g.set_state(b)
         assert g.pruned()
         N = g.nonlinear()
+        # other pencil widths and chunked passes B-D give the same bits
+        g.set_tiles(16, 16, 16)
+        g.set_zchunk(32)
+        N4 = g.nonlinear()
+    assert all(np.array_equal(x, y) for x, y in zip(N, N4))
     No, _ = ol.Oracle(p).nlt(b, b)
     err = rel_l2(N, No)
     say(f"  relative L2 {err:.2e}")
