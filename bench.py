@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
-    ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk]: force the pencil widths (and the z chunk of passes B-D) "
+    ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk[,with_d]]: force the pencil widths (and the z chunk of passes B-D) "
                          "instead of autotuning")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
     ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint", "precision"], default="step",
@@ -337,14 +337,14 @@ def main():
         tv = [int(v) for v in args.tiles.split(",")]
         ctx.set_tiles(*tv[:3])
         if len(tv) > 3:
-            ctx.set_zchunk(tv[3])
+            ctx.set_zchunk(tv[3], len(tv) < 5 or bool(tv[4]))
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.tiles()))
     elif not args.no_autotune:
         # pencil widths of the z-inverse, z-forward and y passes and the z
         # chunk of the y/x/y passes (bit-neutral launch shapes)
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.autotune(3)))
     if tiles is not None:
-        tiles["zchunk"] = ctx.zchunk()
+        tiles["zchunk"], tiles["zchunk_d"] = ctx.zchunk()
     seed = seed0
     seed = ctx.step(args.warmup, seed)
     ctx.sync()
@@ -404,8 +404,12 @@ def main():
         name = lib.PASS_NAMES[dom]
         tb = tr.get("per_launch", {}).get(name)
         if tb is not None and name in ("B_yinv", "C_x_cross", "D_yfwd"):
-            zc_run = (tiles or {}).get("zchunk") or n
-            tb *= zc_run / (tr.get("zchunk") or n)
+            t_run, t_prof = tiles or {}, tr.get("tiles", {"zchunk": tr.get("zchunk"), "zchunk_d": True})
+            def planes(t):
+                if not t.get("zchunk") or (name == "D_yfwd" and not t.get("zchunk_d", True)):
+                    return n
+                return t["zchunk"]
+            tb *= planes(t_run) / planes(t_prof)
         roofline["traffic"] = tb
 
     cpu = None

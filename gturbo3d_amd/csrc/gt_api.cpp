@@ -75,7 +75,8 @@ struct gt_ctx {
   long long fs = 0;
   int prec = 0;
   int resident_blocks = 1024;
-  int zchunk = 0;  // z planes per chunk of the y/x/y passes (0: all at once)
+  int zchunk = 0;   // z planes per chunk of passes B, C (and D) (0: all at once)
+  bool zchunk_d = true;  // pass D runs in the chunks too (else once, after them)
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
   int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
@@ -431,14 +432,16 @@ int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<do
   // memory-side cache (256 MB on MI355X).  The chunk size is a launch shape,
   // picked by gt_autotune (0: all planes at once); results do not depend on it.
   const int zc = (c->zchunk > 0 && c->zchunk < c->nz) ? c->zchunk : c->nz;
+  const bool dch = c->zchunk_d || zc == c->nz;
   for (int z0 = 0; z0 < c->nz; z0 += zc) {
     KP kc = kp;
     kc.z0 = z0;
     kc.zn = std::min(zc, c->nz - z0);
     LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kc, c->work, twp(c, 1), st));
     LAUNCH(PASS_C, launch_xfused(pr, kc, c->work, twp(c, 0), st));
-    LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), st));
+    if (dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), st));
   }
+  if (!dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), st));
   const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
   LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), st));
   return GT_OK;
@@ -1114,7 +1117,8 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
   // passes B-D launch once per z chunk: the mean bytes of one launch
   if (c->zchunk > 0 && c->zchunk < c->nz) {
     const double nch = (double)((c->nz + c->zchunk - 1) / c->zchunk);
-    for (int q : {PASS_B, PASS_C, PASS_D}) by[q] /= nch;
+    for (int q : {PASS_B, PASS_C}) by[q] /= nch;
+    if (c->zchunk_d) by[PASS_D] /= nch;
   }
   for (int q = 0; q < npass && q < NPASS; q++) {
     if (ms) ms[q] = c->prof_ms[q];
@@ -1143,17 +1147,19 @@ int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty) {
   return GT_OK;
 }
 
-int gt_set_zchunk(gt_ctx* c, int planes) {
+int gt_set_zchunk(gt_ctx* c, int planes, int with_d) {
   if (int rc = check_ctx(c)) return rc;
   if (planes < 0) return fail(GT_EINVAL, "gt_set_zchunk: planes must be >= 0");
   c->zchunk = (planes >= c->nz) ? 0 : planes;
+  c->zchunk_d = with_d != 0;
   return GT_OK;
 }
 
-int gt_get_zchunk(gt_ctx* c, int* planes) {
+int gt_get_zchunk(gt_ctx* c, int* planes, int* with_d) {
   if (int rc = check_ctx(c)) return rc;
   if (!planes) return fail(GT_EINVAL, "gt_get_zchunk: null");
   *planes = c->zchunk;
+  if (with_d) *with_d = c->zchunk_d ? 1 : 0;
   return GT_OK;
 }
 
@@ -1226,9 +1232,11 @@ int gt_autotune(gt_ctx* c, int nrep) {
   // z chunk of passes B-D with the chosen widths: wall time of whole
   // evaluations (the chunks add launches, so per-pass sums would flatter them)
   if (rc == GT_OK) {
-    std::vector<int> zcand = {0};
+    // (planes, pass D in the chunks)
+    std::vector<std::pair<int, bool>> zcand = {{0, true}};
     for (int d : {16, 8, 4, 2})
-      if (c->nz % d == 0 && c->nz / d >= 8) zcand.push_back(c->nz / d);
+      if (c->nz % d == 0 && c->nz / d >= 8)
+        for (bool wd : {true, false}) zcand.push_back({c->nz / d, wd});
     hipEvent_t e0 = pool_get(c), e1 = pool_get(c);
     if (!e0 || !e1) rc = fail(GT_EHIP, "autotune events");
     const KP kp = make_kp(c, false);
@@ -1236,7 +1244,8 @@ int gt_autotune(gt_ctx* c, int nrep) {
     std::vector<double> tz(zcand.size(), 1e30);
     for (int round = 0; round < 2; round++)
       for (size_t i = 0; i < zcand.size() && rc == GT_OK; i++) {
-        c->zchunk = zcand[i];
+        c->zchunk = zcand[i].first;
+        c->zchunk_d = zcand[i].second;
         rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
         if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune record");
         for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);
@@ -1249,7 +1258,8 @@ int gt_autotune(gt_ctx* c, int nrep) {
     size_t bi = 0;
     for (size_t i = 1; i < zcand.size(); i++)
       if (tz[i] < tz[bi]) bi = i;
-    c->zchunk = zcand[bi];
+    c->zchunk = zcand[bi].first;
+    c->zchunk_d = zcand[bi].second;
     if (rc == GT_OK) c->tune_ms = tz[bi];  // the tuned time per evaluation
     if (e0) c->pool.push_back(e0);
     if (e1) c->pool.push_back(e1);

@@ -80,8 +80,8 @@ def load():
     i32p = ctypes.POINTER(ctypes.c_int32)
     L.gt_autotune.argtypes = [vp, ip]
     L.gt_set_tiles.argtypes = [vp, ip, ip, ip]
-    L.gt_set_zchunk.argtypes = [vp, ip]
-    L.gt_get_zchunk.argtypes = [vp, ctypes.POINTER(ip)]
+    L.gt_set_zchunk.argtypes = [vp, ip, ip]
+    L.gt_get_zchunk.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip),
                                ctypes.POINTER(ctypes.c_double)]
     L.gt_batch_create.argtypes = [ctypes.POINTER(GtParams), ip, vp, ctypes.POINTER(vp)]
@@ -247,14 +247,16 @@ class Context:
         """Pencil widths of the z-inverse, z-forward and y passes (0: default)."""
         _check(load().gt_set_tiles(self.h, int(tz), int(tze), int(ty)), "gt_set_tiles")
 
-    def set_zchunk(self, planes: int) -> None:
-        """z planes per chunk of the y-inverse, x and y-forward passes (0: all at once)."""
-        _check(load().gt_set_zchunk(self.h, int(planes)), "gt_set_zchunk")
+    def set_zchunk(self, planes: int, with_d: bool = True) -> None:
+        """z planes per chunk of the y-inverse and x passes (0: all at once); with_d:
+        the y-forward pass runs in the chunks too (else once after them)."""
+        _check(load().gt_set_zchunk(self.h, int(planes), 1 if with_d else 0), "gt_set_zchunk")
 
-    def zchunk(self) -> int:
-        v = ctypes.c_int()
-        _check(load().gt_get_zchunk(self.h, ctypes.byref(v)), "gt_get_zchunk")
-        return v.value
+    def zchunk(self):
+        """(planes, with_d)"""
+        v, d = ctypes.c_int(), ctypes.c_int()
+        _check(load().gt_get_zchunk(self.h, ctypes.byref(v), ctypes.byref(d)), "gt_get_zchunk")
+        return v.value, bool(d.value)
 
     def tiles(self):
         tz, tze, ty, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

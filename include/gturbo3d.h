@@ -148,13 +148,15 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
 int gt_autotune(gt_ctx* c, int nrep);
 int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty);
 int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
-/* The y-inverse, x and y-forward passes run in chunks of `planes` z planes
- * (0: all at once), so that each chunk's intermediates are read back while
- * they sit in the device's memory-side cache.  gt_autotune picks it (after
- * the widths, from whole-evaluation times; gt_get_tiles then reports that
- * time).  Results do not depend on it.  No counterpart in the reference. */
-int gt_set_zchunk(gt_ctx* c, int planes);
-int gt_get_zchunk(gt_ctx* c, int* planes);
+/* The y-inverse and x passes (and the y-forward pass when with_d is not 0)
+ * run in chunks of `planes` z planes (0: all at once), so that each chunk's
+ * intermediates are read back while they sit in the device's memory-side
+ * cache; with with_d = 0 the y-forward pass runs once after the chunks.
+ * gt_autotune picks both (after the widths, from whole-evaluation times;
+ * gt_get_tiles then reports that time).  Results do not depend on them.
+ * with_d may be NULL in gt_get_zchunk.  No counterpart in the reference. */
+int gt_set_zchunk(gt_ctx* c, int planes, int with_d);
+int gt_get_zchunk(gt_ctx* c, int* planes, int* with_d);
 
 /* Batches: nmembers independent members on one grid with the same
  * parameters except omega (omegas[m], or p->omega when omegas is NULL) and

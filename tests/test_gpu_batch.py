@@ -114,16 +114,16 @@ def test_tile_widths_do_not_change_results(require_gpu, n):
                     results[(tz, tze, ty)] = g.get_state()
         # z chunks of passes B-D: equal chunks, a ragged last chunk, one plane
         g.set_tiles(tz0, tze0, ty0)
-        for zc in (n // 4, 3 * n // 8 + 1, 1):
-            g.set_zchunk(zc)
-            assert g.zchunk() == zc
+        for zc, wd in ((n // 4, True), (3 * n // 8 + 1, True), (1, True), (n // 4, False)):
+            g.set_zchunk(zc, wd)
+            assert g.zchunk() == (zc, wd)
             g.set_state(b)
             g.step(2, 3)
-            results[("zchunk", zc)] = g.get_state()
+            results[("zchunk", zc, wd)] = g.get_state()
         g.set_zchunk(0)
         tz, tze, ty, ms = g.autotune(2)
         assert tz in (8, 16) and tze in (8, 16) and ty in (8, 16) and ms > 0
-        assert g.zchunk() in (0, n // 16, n // 8, n // 4, n // 2)
+        assert g.zchunk()[0] in (0, n // 16, n // 8, n // 4, n // 2)
     ref = results[(tz0, tze0, ty0)]
     for key, st in results.items():
         for x, y in zip(st, ref):
