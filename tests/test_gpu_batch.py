@@ -69,6 +69,10 @@ def test_ensemble_graph_replays_match_eager(require_gpu):
     ics_b = [initial.initial_state(pb, seed=30 + m) for m in range(2)]
     A = lib.Batch(pa, 3, om_a)
     B = lib.Batch(pb, 2, om_b)
+    # passes B-D of batch A in z chunks (a ragged last one), pass D of B whole:
+    # the captured graphs hold every chunk launch
+    A.set_zchunk(12)
+    B.set_zchunk(8, False)
     for m in range(3):
         A.set_member(m, ics_a[m])
     for m in range(2):
