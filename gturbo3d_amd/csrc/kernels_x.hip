@@ -122,8 +122,8 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
     for (int k = t + P * (w / 3); k <= kc; k += 2 * P) {
       const int kk = (k == 0) ? 0 : N - k;
       const cx<W> Y = lds[c * N + k], Yc = lds[c * N + kk];
-      st_row(d0, (unsigned)k, cx<W>{(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)});
-      st_row(d1, (unsigned)k, cx<W>{(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)});
+      st_row_p<NT_C>(d0, (unsigned)k, cx<W>{(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)});
+      st_row_p<NT_C>(d1, (unsigned)k, cx<W>{(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)});
     }
   }
 }

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
       bool ld = on && F::in_ok(t, m);
       if constexpr (SIGN > 0) ld = ld && keep_idx(n, N, Ky);
       cx<W> v = {W(0), W(0)};
-      if (ld) v = ld_row(tile + (long long)(P * m + r * F::NR0) * ys, lane);
+      if (ld) v = ld_row_p<(SIGN < 0 ? NT_D : NT_B)>(tile + (long long)(P * m + r * F::NR0) * ys, lane);
       in[0][m * R0 + r] = v;
     }
   F::run(lay, lds, tw, t, in, out);
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
       if constexpr (SIGN < 0) {
         if (!keep_idx(n, N, Ky)) continue;
       }
-      st_row(tile + (long long)(P * m + r * F::NRL) * ys, lane, out[0][m * RL + r]);
+      st_row_p<(SIGN < 0 ? NT_D : NT_B)>(tile + (long long)(P * m + r * F::NRL) * ys, lane, out[0][m * RL + r]);
     }
   }
 }

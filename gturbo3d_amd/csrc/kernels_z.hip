@@ -150,7 +150,7 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   for (int m = 0; m < ML; m++)
     if (F::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < RL; r++) st_row(dst + (long long)(P * m + r * F::NRL) * zs, lane, out[0][m * RL + r]);
+      for (int r = 0; r < RL; r++) st_row_p<NT_A>(dst + (long long)(P * m + r * F::NRL) * zs, lane, out[0][m * RL + r]);
 }
 
 // VS: simple viscosity, the integrating factor from the per-axis tables
@@ -215,7 +215,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 #pragma unroll
       for (int r = 0; r < R0; r++) {
         cx<W> v = {W(0), W(0)};
-        if (K >= 0 && F::in_ok(t, m)) v = ld_row(src + (long long)(P * m + r * F::NR0) * zs, lane);
+        if (K >= 0 && F::in_ok(t, m)) v = ld_row_p<NT_E>(src + (long long)(P * m + r * F::NR0) * zs, lane);
         in[0][m * R0 + r] = v;
       }
     F::run(lay, lds, tw, t, in, o);
@@ -284,7 +284,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
       }
       if constexpr (EPI == EPI_NONLIN || EPI == EPI_INIT) {
 #pragma unroll
-        for (int f = 0; f < 3; f++) st_row(out + f * kp.fs + c, lane, nl[f]);
+        for (int f = 0; f < 3; f++) st_row_p<NT_E>(out + f * kp.fs + c, lane, nl[f]);
       } else {
         const double e = VS ? exy * kp.emz[n] : emk_of(kp, k2);
 #pragma unroll
@@ -294,7 +294,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
           if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
           else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
           if (k0) o = {0.0, 0.0};
-          st_row(out + f * kp.fs + c, lane, o);
+          st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
         }
       }
     }
