@@ -103,6 +103,7 @@ struct gt_ctx {
 
   // truncation extents
   int kxcut = 0;
+  int kycut = 0, kzcut = 0;  // largest kyext / kzext entries
   std::vector<int16_t> kyext, kzext;
   bool pruned = false;
 
@@ -303,6 +304,10 @@ void build_extents(gt_ctx* c) {
   for (int i = 0; i < nxh; i++)
     if (c->kyext[i] >= 0) c->kxcut = i;
   if (c->kxcut < 0) c->kxcut = 0;
+  c->kycut = 0;
+  c->kzcut = 0;
+  for (int i = 0; i < nxh; i++) c->kycut = std::max(c->kycut, (int)c->kyext[i]);
+  for (int16_t K : c->kzext) c->kzcut = std::max(c->kzcut, (int)K);
 }
 
 // Stockham twiddles exp(-2 pi i r k/(Ns R)) per stage (fft_core.hpp)
@@ -345,6 +350,8 @@ KP make_kp(const gt_ctx* c, bool dense) {
   const bool pr = c->pruned && !dense;
   kp.pruned = pr ? 1 : 0;
   kp.kxcut = pr ? c->kxcut : c->nxh - 1;
+  kp.kycut = pr ? c->kycut : c->ny / 2;
+  kp.kzcut = pr ? c->kzcut : c->nz / 2;
   kp.kyext = c->dkyext;
   kp.kzext = c->dkzext;
   kp.omega = c->p.omega;

@@ -71,6 +71,21 @@ template <typename V>
 __device__ __forceinline__ void st_row(V* row, unsigned i, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(row) + i * (unsigned)sizeof(V)) = v;
 }
+// A thread slot of a transform of length N with P threads per sequence
+// covers positions t + C, 0 <= t < P.  With every mode index min(n, N - n)
+// of the spectral side at most a cut below 3N/8, a slot whose positions all
+// lie in [3N/8, 5N/8) holds only zeros on input (inverse transforms) or
+// values that are never stored (forward transforms): it is pruned at
+// compile time, so the butterflies fold the zeros or drop the dead outputs.
+template <int N, int P, int C>
+__host__ __device__ constexpr bool mid_slot() {
+  return 8 * C >= 3 * N && 8 * (C + P) <= 5 * N;
+}
+// lengths that get pruned instantiations (the large grids, where the pencil
+// passes' arithmetic counts)
+__host__ __device__ constexpr bool pz_len(int n) { return n >= 256; }
+__host__ __device__ constexpr bool pz_cut(int n, int cut) { return 8 * cut < 3 * n; }
+
 // Non-temporal loads and stores per pass, as bit masks over the passes.
 // Measured at 512^3: non-temporal stores of every pass's output are faster
 // (2% per step in FP64 and FP32 work for passes A and D, 2% more for B and

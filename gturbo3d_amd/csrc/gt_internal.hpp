@@ -32,6 +32,7 @@ struct KP {
   int truncate;
   int pruned;
   int kxcut;
+  int kycut, kzcut;  // largest kyext and kzext entries (ny/2, nz/2 in dense mode)
   const int16_t* kyext;
   const int16_t* kzext;
   double omega, dt, dt2, nu, mu, alpnu, alpmu;

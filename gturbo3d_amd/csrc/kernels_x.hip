@@ -37,10 +37,6 @@ namespace gt {
 #ifndef GT_XFUSED_WAVES_PER_EU
 #define GT_XFUSED_WAVES_PER_EU 1
 #endif
-template <int N, int P, int C>
-__host__ __device__ constexpr bool mid_slot() {
-  return 8 * C >= 3 * N && 8 * (C + P) <= 5 * N;
-}
 template <typename W, int N, int P, bool PZ>
 __global__ __launch_bounds__(6 * P) __attribute__((amdgpu_waves_per_eu(GT_XFUSED_WAVES_PER_EU)))
 void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
@@ -244,7 +240,7 @@ template <typename W, int N>
 static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = row_P(N);
   dim3 grid(kp.ny / 2, kp.zn, kp.nmem);
-  if (8 * kp.kxcut < 3 * N)
+  if (pz_cut(N, kp.kxcut))
     k_xfused<W, N, P, true><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   else
     k_xfused<W, N, P, false><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);

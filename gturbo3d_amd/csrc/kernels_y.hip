@@ -10,7 +10,9 @@
 
 namespace gt {
 
-template <typename W, int N, int T, int P, int SIGN>
+// PZ: the slots that mid_slot() prunes (kycut < 3N/8) are zeros known at
+// compile time on input (SIGN > 0) or never stored on output (SIGN < 0).
+template <typename W, int N, int T, int P, int SIGN, bool PZ = false>
 __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* __restrict__ tw, int nfields) {
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, SIGN, LI>;
@@ -32,41 +34,47 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   LI lay{p};
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
-#pragma unroll
-  for (int m = 0; m < M0; m++)
-#pragma unroll
-    for (int r = 0; r < R0; r++) {
+  static_for<0, M0 * R0>([&](auto qc) {
+    constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+    cx<W> v = {W(0), W(0)};
+    if constexpr (!(PZ && SIGN > 0 && mid_slot<N, P, P * m + r * F::NR0>())) {
       const int n = F::in_pos(t, m, r);
       bool ld = on && F::in_ok(t, m);
       if constexpr (SIGN > 0) ld = ld && keep_idx(n, N, Ky);
-      cx<W> v = {W(0), W(0)};
       if (ld) v = ld_row_p<(SIGN < 0 ? NT_D : NT_B)>(tile + (long long)(P * m + r * F::NR0) * ys, lane);
-      in[0][m * R0 + r] = v;
     }
+    in[0][qs] = v;
+  });
   F::run(lay, lds, tw, t, in, out);
   if (!on) return;
-#pragma unroll
-  for (int m = 0; m < ML; m++) {
-    if (!F::out_ok(t, m)) continue;
-#pragma unroll
-    for (int r = 0; r < RL; r++) {
+  static_for<0, ML * RL>([&](auto qc) {
+    constexpr int qs = decltype(qc)::value, m = qs / RL, r = qs % RL;
+    if constexpr (!(PZ && SIGN < 0 && mid_slot<N, P, P * m + r * F::NRL>())) {
+      if (!F::out_ok(t, m)) return;
       const int n = F::out_pos(t, m, r);
       if constexpr (SIGN < 0) {
-        if (!keep_idx(n, N, Ky)) continue;
+        if (!keep_idx(n, N, Ky)) return;
       }
-      st_row_p<(SIGN < 0 ? NT_D : NT_B)>(tile + (long long)(P * m + r * F::NRL) * ys, lane, out[0][m * RL + r]);
+      st_row_p<(SIGN < 0 ? NT_D : NT_B)>(tile + (long long)(P * m + r * F::NRL) * ys, lane, out[0][qs]);
     }
-  }
+  });
 }
 
 template <typename W, int N, int T>
 static hipError_t yfft_nt(int sign, int nfields, const KP& kp, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
   dim3 grid((kp.kxcut + T) / T, kp.zn, nfields * kp.nmem);
-  if (sign > 0)
-    k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
-  else
-    k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw, nfields);
+  cx<W>* w = (cx<W>*)work;
+  const cx<W>* t = (const cx<W>*)tw;
+  if constexpr (pz_len(N)) {
+    if (kp.pruned && pz_cut(N, kp.kycut)) {
+      if (sign > 0) k_yfft<W, N, T, P, +1, true><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
+      else k_yfft<W, N, T, P, -1, true><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
+      return hipGetLastError();
+    }
+  }
+  if (sign > 0) k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
+  else k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
   return hipGetLastError();
 }
 

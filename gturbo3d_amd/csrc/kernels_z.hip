@@ -35,7 +35,9 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 //   (src/produ.f90:48-53);  w_f = (fkxs + fkys + fkzs) b_f (src/nlt.f90:30-36)
 // EMK: transform a = Z emk b instead of b (the stage-1 state bh of a step in
 // the reference dt2 mode, formed as it is loaded; Z sets k = 0 to zero)
-template <typename W, int N, int T, int P, bool EMK = false>
+// PZ: the slots that mid_slot() prunes (kzcut < 3N/8) are zeros, known at
+// compile time.
+template <typename W, int N, int T, int P, bool EMK = false, bool PZ = false>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
                                                const cx<W>* __restrict__ tw, int nout) {
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
@@ -94,30 +96,31 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   constexpr bool BATCH = sizeof(W) == 8;
   cx<double> la[M0 * R0], lb[M0 * R0];
   if constexpr (BATCH) {
-#pragma unroll
-    for (int m = 0; m < M0; m++)
-#pragma unroll
-      for (int r = 0; r < R0; r++) {
+    static_for<0, M0 * R0>([&](auto qc) {
+      constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+      if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NR0>())) {
         const int n = F::in_pos(t, m, r);
         // an unused slot reads element c of the column instead
         const unsigned o = (F::in_ok(t, m) && keep_idx(n, N, K)) ? lane : (unsigned)p;
         const long long c = (long long)(P * m + r * F::NR0) * zs;
-        la[m * R0 + r] = ld_row(A + c, o);
+        la[qs] = ld_row(A + c, o);
         // curl is uniform per block: the k^2 blocks skip the second field
-        lb[m * R0 + r] = curl ? ld_row(B + c, o) : cx<double>{0.0, 0.0};
+        lb[qs] = curl ? ld_row(B + c, o) : cx<double>{0.0, 0.0};
       }
+    });
   }
-#pragma unroll
-  for (int m = 0; m < M0; m++)
-#pragma unroll
-    for (int r = 0; r < R0; r++) {
+  static_for<0, M0 * R0>([&](auto qc) {
+    constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+    if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
+      in[0][qs] = {W(0), W(0)};
+    } else {
       const int n = F::in_pos(t, m, r);
       const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
       const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
       cx<double> va = {0.0, 0.0}, vb = {0.0, 0.0};
       if constexpr (BATCH) {
-        va = use ? la[m * R0 + r] : cx<double>{0.0, 0.0};
-        vb = (use && curl) ? lb[m * R0 + r] : cx<double>{0.0, 0.0};
+        va = use ? la[qs] : cx<double>{0.0, 0.0};
+        vb = (use && curl) ? lb[qs] : cx<double>{0.0, 0.0};
       } else if (use) {
         const long long c = (long long)(P * m + r * F::NR0) * zs;
         va = ld_row(A + c, lane);
@@ -140,8 +143,9 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         re = k2 * va.x;
         im = k2 * va.y;
       }
-      in[0][m * R0 + r] = {(W)re, (W)im};
+      in[0][qs] = {(W)re, (W)im};
     }
+  });
   LI lay{p};
   F::run(lay, lds, tw, t, in, out);
   if (K < 0) return;
@@ -165,7 +169,9 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
 #ifndef GT_ZFWD_WPE32
 #define GT_ZFWD_WPE32 4
 #endif
-template <typename W, int N, int T, int P, int EPI, bool VS = false>
+// PZ: the output slots that mid_slot() prunes (kzcut < 3N/8) are never
+// stored, so the transform's last stage drops them.
+template <typename W, int N, int T, int P, int EPI, bool VS = false, bool PZ = false>
 __global__ __launch_bounds__(T* P)
 __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE32))) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
                                                cx<double>* out, const cx<W>* __restrict__ tw) {
@@ -229,19 +235,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   const double fxs = fx * fx, fys = fy * fy;
   const double kxn = kp.fkxn[kxc], kyn = kp.fkyn[j];
   const double exy = VS ? kp.emx[kxc] * kp.emy[j] : 0.0;
-#pragma unroll
-  for (int m = 0; m < ML; m++) {
-    if (!F::out_ok(t, m)) continue;
-#pragma unroll
-    for (int r = 0; r < RL; r++) {
+  static_for<0, ML * RL>([&](auto qc) {
+    constexpr int qs = decltype(qc)::value, m = qs / RL, r = qs % RL;
+    if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NRL>())) {
+      if (!F::out_ok(t, m)) return;
       const int n = F::out_pos(t, m, r);
-      if (!keep_idx(n, N, K)) continue;
+      if (!keep_idx(n, N, K)) return;
       const long long c = tile + (long long)(P * m + r * F::NRL) * zs;  // + lane
       const double fz = kp.fkz[n];
       const bool k0 = (kx == 0) && (j == 0) && (n == 0);
       cx<double> a[3], bv[3];
 #pragma unroll
-      for (int f = 0; f < 3; f++) a[f] = {(double)y[f][m * RL + r].x * kp.scale, (double)y[f][m * RL + r].y * kp.scale};
+      for (int f = 0; f < 3; f++) a[f] = {(double)y[f][qs].x * kp.scale, (double)y[f][qs].y * kp.scale};
       const double k2 = fxs + fys + fz * fz;
       if constexpr (EPI == EPI_INIT) {
         // incompr (src/trunc.f90:47-63), k = 0 skipped
@@ -298,7 +303,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
         }
       }
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -308,6 +313,12 @@ template <typename W, int N, int T>
 static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
   const int grid = (kp.kxcut + T) / T * kp.ny * nout * kp.nmem;
+  if constexpr (pz_len(N)) {
+    if (kp.pruned && pz_cut(N, kp.kzcut)) {
+      k_zinv<W, N, T, P, false, true><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
+      return hipGetLastError();
+    }
+  }
   k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
   return hipGetLastError();
 }
@@ -330,6 +341,19 @@ static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b
   cx<double>* o = (cx<double>*)out;
   const cx<W>* t = (const cx<W>*)tw;
   const bool vs = kp.emx != nullptr;
+  if constexpr (pz_len(N)) {
+    // the step epilogues of the pruned step: dropped output slots
+    if (kp.pruned && pz_cut(N, kp.kzcut) && (epi == EPI_STEP1 || epi == EPI_STEP2)) {
+      if (epi == EPI_STEP1) {
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      } else {
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      }
+      return hipGetLastError();
+    }
+  }
   switch (epi) {
     case EPI_STEP1:
       if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
