@@ -21,6 +21,9 @@ namespace gt {
 
 enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 
+#ifndef GT_ZINV_PAIR
+#define GT_ZINV_PAIR 1
+#endif
 #ifndef GT_ZINV_XCD
 #define GT_ZINV_XCD 1
 #endif
@@ -37,9 +40,13 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 // the reference dt2 mode, formed as it is loaded; Z sets k = 0 to zero)
 // PZ: the slots that mid_slot() prunes (kzcut < 3N/8) are zeros, known at
 // compile time.
-template <typename W, int N, int T, int P, bool EMK = false, bool PZ = false>
+// PAIR: nout = 3 blocks per column, block q computing u_q and then w_fb,
+// the k^2 component of the second field u_q reads (u_0 + w_1, u_1 + w_2,
+// u_2 + w_0): that field is loaded once for both transforms.
+template <typename W, int N, int T, int P, bool EMK = false, bool PZ = false, bool PAIR = false>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
                                                const cx<W>* __restrict__ tw, int nout) {
+  static_assert(!(PAIR && EMK), "the paired blocks transform b itself");
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, +1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
@@ -148,13 +155,43 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   });
   LI lay{p};
   F::run(lay, lds, tw, t, in, out);
-  if (K < 0) return;
-  cx<W>* dst = work + q * kp.fs + tile;
+  auto store = [&](int comp) {
+    cx<W>* dst = work + comp * kp.fs + tile;
 #pragma unroll
-  for (int m = 0; m < ML; m++)
-    if (F::out_ok(t, m))
+    for (int m = 0; m < ML; m++)
+      if (F::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < RL; r++) st_row_p<NT_A>(dst + (long long)(P * m + r * F::NRL) * zs, lane, out[0][m * RL + r]);
+        for (int r = 0; r < RL; r++) st_row_p<NT_A>(dst + (long long)(P * m + r * F::NRL) * zs, lane, out[0][m * RL + r]);
+  };
+  if constexpr (PAIR) {
+    // every thread takes part in the second transform's exchanges, so the
+    // idle pencils skip the stores instead of returning
+    if (K >= 0) store(q);
+    static_for<0, M0 * R0>([&](auto qc) {
+      constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+      if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
+        in[0][qs] = {W(0), W(0)};
+      } else {
+        const int n = F::in_pos(t, m, r);
+        const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
+        const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
+        cx<double> vb = {0.0, 0.0};
+        if constexpr (BATCH) {
+          vb = use ? lb[qs] : cx<double>{0.0, 0.0};
+        } else if (use) {
+          vb = ld_row(B + (long long)(P * m + r * F::NR0) * zs, lane);
+        }
+        const double k2 = fx * fx + fy * fy + fz * fz;
+        in[0][qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
+      }
+    });
+    F::run(lay, lds, tw, t, in, out);
+    if (K < 0) return;
+    store(3 + fb);
+  } else {
+    if (K < 0) return;
+    store(q);
+  }
 }
 
 // VS: simple viscosity, the integrating factor from the per-axis tables
@@ -312,14 +349,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 template <typename W, int N, int T>
 static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
-  const int grid = (kp.kxcut + T) / T * kp.ny * nout * kp.nmem;
+  // all six outputs: three blocks per column, each a curl and a k^2 component
+  const bool pair = GT_ZINV_PAIR && nout == 6;
+  const int nb = pair ? 3 : nout;
+  const int grid = (kp.kxcut + T) / T * kp.ny * nb * kp.nmem;
+  const cx<double>* bb = (const cx<double>*)b;
+  cx<W>* w = (cx<W>*)work;
+  const cx<W>* t = (const cx<W>*)tw;
   if constexpr (pz_len(N)) {
     if (kp.pruned && pz_cut(N, kp.kzcut)) {
-      k_zinv<W, N, T, P, false, true><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
+      if (pair) k_zinv<W, N, T, P, false, true, GT_ZINV_PAIR != 0><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
+      else k_zinv<W, N, T, P, false, true><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
       return hipGetLastError();
     }
   }
-  k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<W>*)work, (const cx<W>*)tw, nout);
+  if (pair) k_zinv<W, N, T, P, false, false, GT_ZINV_PAIR != 0><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
+  else k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
   return hipGetLastError();
 }
 
