@@ -71,6 +71,27 @@ template <typename V>
 __device__ __forceinline__ void st_row(V* row, unsigned i, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(row) + i * (unsigned)sizeof(V)) = v;
 }
+// Entry p (0 <= p < T) of T consecutive int16 extents that start at a
+// block-uniform, 4-byte aligned address.  They are read through the constant
+// address space, so the loads are scalar and hit the small per-CU-group
+// constant cache; a vector load instead would be one full memory latency
+// that every dependent load of the block waits behind.
+template <int T>
+__device__ __forceinline__ int tile_ext(const int16_t* e, int p) {
+  static_assert(T % 2 == 0, "pairs of entries per 32-bit word");
+  typedef __attribute__((address_space(4))) const int CInt;
+  CInt* e32 = (CInt*)(e);
+  int v = -1;
+#pragma unroll
+  for (int i = 0; i < T / 2; i++) {
+    const int w = e32[i];
+    const int lo = (int)(int16_t)(w & 0xffff), hi = w >> 16;  // entries 2i, 2i+1
+    v = (p == 2 * i) ? lo : v;
+    v = (p == 2 * i + 1) ? hi : v;
+  }
+  return v;
+}
+
 // A thread slot of a transform of length N with P threads per sequence
 // covers positions t + C, 0 <= t < P.  With every mode index min(n, N - n)
 // of the spectral side at most a cut below 3N/8, a slot whose positions all

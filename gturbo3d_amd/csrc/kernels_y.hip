@@ -24,7 +24,14 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   const int f = blockIdx.z % nfields;
   work += (blockIdx.z / nfields) * kp.wstride;
   const bool on = kx <= kp.kxcut;
-  const int Ky = on ? (kp.pruned ? (int)kp.kyext[kx] : (N >> 1)) : -1;
+  // inverse: every load depends on Ky, so the extents come through scalar
+  // loads; forward: Ky only guards the stores, and a vector load measured faster
+  int ke = N >> 1;
+  if (kp.pruned) {
+    if constexpr (SIGN > 0) ke = tile_ext<T>(kp.kyext + blockIdx.x * T, p);
+    else ke = on ? (int)kp.kyext[kx] : -1;
+  }
+  const int Ky = on ? ke : -1;
   const int ys = kp.nxp;
   // element n = t + c of the thread's pencil (c = P*m + r*N/R, a constant of
   // the slot) sits at tile + c*ys + lane: a start uniform over the block and
