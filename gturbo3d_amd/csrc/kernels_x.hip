@@ -34,6 +34,9 @@ namespace gt {
 // [3N/8, 5N/8) holds only modes above kxcut, so its inverse input is zero
 // (no loads) and its forward output is never unpacked (not stored, and the
 // butterfly outputs feeding it are dropped).
+#ifndef GT_XFUSED_REGU
+#define GT_XFUSED_REGU 1
+#endif
 #ifndef GT_XFUSED_WAVES_PER_EU
 #define GT_XFUSED_WAVES_PER_EU 1
 #endif
@@ -56,6 +59,12 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   const int kc = kp.kxcut;
   const L lay{w, 0};
 
+  // With the inverse's output positions equal to the forward's input
+  // positions (same first and last radix), wave w keeps its u_w in registers
+  // and computes product c = w + 2 (mod 3), whose first term is u_w w_c2:
+  // three reads from LDS per position instead of four.
+  constexpr bool REGU = GT_XFUSED_REGU && FI::RL == FF::R0 && FI::ML == FF::M0;
+  cx<W> uo[1][FI::EL];
   // inverse transform of field w (u_x, u_y, u_z, w_x, w_y, w_z), rows y0 and y0+1
   {
     cx<W> in[1][FI::E0], out[1][FI::EL];
@@ -76,12 +85,18 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
       if (FI::out_ok(t, m))
 #pragma unroll
         for (int r = 0; r < FI::RL; r++) lds[w * N + FI::out_pos(t, m, r)] = out[0][m * FI::RL + r];
+    if constexpr (REGU) {
+#pragma unroll
+      for (int q = 0; q < FI::EL; q++) uo[0][q] = out[0][q];
+    }
   }
   __syncthreads();
   // u x w (src/produ.f90:14-16 with a = u, b = w), component w, both rows
   cx<W> fin[1][FF::E0], fout[1][FF::EL];
   const int comp = (w < 3) ? w : w - 3;
-  const int c1 = (comp + 1) % 3, c2 = (comp + 2) % 3;
+  // the product component this group forms and transforms (groups 0-2)
+  const int pc = REGU ? (comp + 2) % 3 : comp;
+  const int c1 = (pc + 1) % 3, c2 = (pc + 2) % 3;
 #pragma unroll
   for (int m = 0; m < FF::M0; m++)
 #pragma unroll
@@ -92,7 +107,9 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
         // (u x w)_c = u_c1 w_c2 - u_c2 w_c1 with c1 = c+1, c2 = c+2 (mod 3):
         // one code path with computed rows (three branches on c would keep
         // registers for all of them)
-        const cx<W> u1 = lds[c1 * N + o], u2 = lds[c2 * N + o];
+        // u_c1 is this wave's own field when REGU (c1 = w)
+        const cx<W> u1 = REGU ? uo[0][m * FF::R0 + r] : lds[c1 * N + o];
+        const cx<W> u2 = lds[c2 * N + o];
         const cx<W> w1 = lds[(3 + c1) * N + o], w2 = lds[(3 + c2) * N + o];
         v = {u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y};
       }
@@ -113,11 +130,15 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   // unpack the two rows of component c = w % 3 (groups c and c+3 share it)
   {
     const int c = comp;
+    // group ca transformed product c (c = ca + 2 mod 3 when REGU) and left
+    // it in its own rows: the transforms of groups 0-2 run unsynchronised,
+    // each exchanging through its own rows only
+    const int ca = REGU ? (c + 1) % 3 : c;
     cx<W>* d0 = work + c * kp.fs + r0;
     cx<W>* d1 = d0 + kp.nxp;
     for (int k = t + P * (w / 3); k <= kc; k += 2 * P) {
       const int kk = (k == 0) ? 0 : N - k;
-      const cx<W> Y = lds[c * N + k], Yc = lds[c * N + kk];
+      const cx<W> Y = lds[ca * N + k], Yc = lds[ca * N + kk];
       st_row_p<NT_C>(d0, (unsigned)k, cx<W>{(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)});
       st_row_p<NT_C>(d1, (unsigned)k, cx<W>{(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)});
     }
