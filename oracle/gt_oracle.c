@@ -24,10 +24,13 @@
  *   3D R2C / C2R            numpy.fft.rfftn / irfftn (pocketfft), direct DFT
  *   curl / invcurl / cross  analytic fields transformed by the oracle FFT
  *   incompr                 divergence-free output, idempotence
- *   nlt                     Beltrami (ABC) fields, energy / helicity invariants
+ *   nlt                     closed forms: two shear modes, Coriolis on the
+ *                           second argument (test_oracle_nlt_closed_form.py);
+ *                           Beltrami (ABC) fields, energy / helicity invariants
  *   step1 / step2           linear decay closed form exp(-nu k^2 dt)
  *   forcing                 orthogonality to k, Hermitian pairs, amplitude
- *   step (composition)      composition of the separately pinned pieces
+ *   step (composition)      rotating ABC closed form (both stage-1 modes);
+ *                           composition of the separately pinned pieces
  *   diagnostics             Parseval against physical-space sums
  *
  * Layout: a complex field is complex128 (NX/2+1, NY, NZ) column-major, i.e.
@@ -630,12 +633,38 @@ void or_incompr(or_ctx *c, double *ax, double *ay, double *az) {
 /* ------------------------------------------------------------------ */
 
 /*
+ * Mutation switches (OR_MUTANT, 0 = none, the only value the Makefile uses):
+ * each plants one plausible misreading of the reference in nlt or the step,
+ * so that tests/test_oracle_mutants.py can build the oracle once per value
+ * and show that the closed-form tests reject every one of them.
+ *   1  vorticity |k| a instead of k^2 a        (nlt.f90:26-40)
+ *   2  w x u instead of u x w                  (nlt.f90:55, produ.f90:14-16)
+ *   3  vorticity -k^2 a                        (nlt.f90:26-40)
+ *   4  Coriolis -i omega kz b                  (nlt.f90:72-77)
+ *   5  Coriolis on the first argument a        (nlt.f90:72-77)
+ *   6  stage-2 Coriolis on bh, not the old b   (main.f90:144)
+ *   7  step2 as emk^2 a + dt c                 (rkstep.f90:86-96)
+ */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+
+/*
  * nlt(a, v, b) (nlt.f90:1-95): on return a = T[invcurl(F[u x w] + i omega kz b)]
  * and v = F[u x w], with u = F^-1[curl a], w = F^-1[k^2 a].
  */
 void or_nlt(or_ctx *c, double *ax, double *ay, double *az, double *vx, double *vy, double *vz,
             const double *bx, const double *by, const double *bz) {
   const size_t m = (size_t)c->nxh * c->ny * c->nz;
+#if OR_MUTANT == 5
+  double *a0 = malloc(sizeof(double) * 6 * m);
+  memcpy(a0, ax, sizeof(double) * 2 * m);
+  memcpy(a0 + 2 * m, ay, sizeof(double) * 2 * m);
+  memcpy(a0 + 4 * m, az, sizeof(double) * 2 * m);
+  bx = a0;
+  by = a0 + 2 * m;
+  bz = a0 + 4 * m;
+#endif
   memcpy(vx, ax, sizeof(double) * 2 * m);
   memcpy(vy, ay, sizeof(double) * 2 * m);
   memcpy(vz, az, sizeof(double) * 2 * m);
@@ -646,6 +675,11 @@ void or_nlt(or_ctx *c, double *ax, double *ay, double *az, double *vx, double *v
       for (int i = 0; i < c->nxh; i++) {
         size_t q = IDX(c, i, j, k);
         double k2 = c->fkxs[i] + c->fkys[j] + c->fkzs[k];
+#if OR_MUTANT == 1
+        k2 = sqrt(k2);
+#elif OR_MUTANT == 3
+        k2 = -k2;
+#endif
         RE(ax, q) = k2 * RE(ax, q); IM(ax, q) = k2 * IM(ax, q);
         RE(ay, q) = k2 * RE(ay, q); IM(ay, q) = k2 * IM(ay, q);
         RE(az, q) = k2 * RE(az, q); IM(az, q) = k2 * IM(az, q);
@@ -656,11 +690,22 @@ void or_nlt(or_ctx *c, double *ax, double *ay, double *az, double *vx, double *v
   or_fft_inv(c, ax);
   or_fft_inv(c, ay);
   or_fft_inv(c, az);
+#if OR_MUTANT == 2
+  or_cross(c, ax, ay, az, vx, vy, vz);
+  memcpy(vx, ax, sizeof(double) * 2 * m);
+  memcpy(vy, ay, sizeof(double) * 2 * m);
+  memcpy(vz, az, sizeof(double) * 2 * m);
+#else
   or_cross(c, vx, vy, vz, ax, ay, az);
+#endif
   or_fft_dir(c, vx);
   or_fft_dir(c, vy);
   or_fft_dir(c, vz);
+#if OR_MUTANT == 4
+  const double om = -c->p.omega;
+#else
   const double om = c->p.omega;
+#endif
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < c->nz; k++)
     for (int j = 0; j < c->ny; j++)
@@ -680,6 +725,9 @@ void or_nlt(or_ctx *c, double *ax, double *ay, double *az, double *vx, double *v
     or_trunc(c, ay);
     or_trunc(c, az);
   }
+#if OR_MUTANT == 5
+  free(a0);
+#endif
 }
 
 /* step1, live branch (rkstep.f90:30-48): c <- emk (a + dt2 c), c(0) = 0 */
@@ -707,9 +755,15 @@ void or_step2(or_ctx *c, double *ax, double *ay, double *az, const double *cx, c
 #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < m; q++) {
     double e = c->emk[q];
+#if OR_MUTANT == 7
+    RE(ax, q) = e * e * RE(ax, q) + dt * RE(cx, q); IM(ax, q) = e * e * IM(ax, q) + dt * IM(cx, q);
+    RE(ay, q) = e * e * RE(ay, q) + dt * RE(cy, q); IM(ay, q) = e * e * IM(ay, q) + dt * IM(cy, q);
+    RE(az, q) = e * e * RE(az, q) + dt * RE(cz, q); IM(az, q) = e * e * IM(az, q) + dt * IM(cz, q);
+#else
     RE(ax, q) = e * (e * RE(ax, q) + dt * RE(cx, q)); IM(ax, q) = e * (e * IM(ax, q) + dt * IM(cx, q));
     RE(ay, q) = e * (e * RE(ay, q) + dt * RE(cy, q)); IM(ay, q) = e * (e * IM(ay, q) + dt * IM(cy, q));
     RE(az, q) = e * (e * RE(az, q) + dt * RE(cz, q)); IM(az, q) = e * (e * IM(az, q) + dt * IM(cz, q));
+#endif
   }
   RE(ax, 0) = IM(ax, 0) = 0.0;
   RE(ay, 0) = IM(ay, 0) = 0.0;
@@ -800,7 +854,16 @@ void or_step(or_ctx *c, double *bx, double *by, double *bz, double *bnx, double 
     memcpy(bnz, bz, sizeof(double) * 2 * m);
     or_nlt(c, bnx, bny, bnz, ux, uy, uz, bx, by, bz);
     or_step1(c, bx, by, bz, bnx, bny, bnz);
+#if OR_MUTANT == 6
+    double *bh = malloc(sizeof(double) * 6 * m);
+    memcpy(bh, bnx, sizeof(double) * 2 * m);
+    memcpy(bh + 2 * m, bny, sizeof(double) * 2 * m);
+    memcpy(bh + 4 * m, bnz, sizeof(double) * 2 * m);
+    or_nlt(c, bnx, bny, bnz, ux, uy, uz, bh, bh + 2 * m, bh + 4 * m);
+    free(bh);
+#else
     or_nlt(c, bnx, bny, bnz, ux, uy, uz, bx, by, bz);
+#endif
     or_step2(c, bx, by, bz, bnx, bny, bnz);
     or_forcing(c, bx, by, bz, seed);
   }

@@ -27,19 +27,20 @@ class _OrParams(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path: str | None = None):
+    """The oracle library at ``path`` (default: oracle/liboracle.so, built on demand)."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build_oracle()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
         L.or_create.restype = vp
         L.or_create.argtypes = [ctypes.POINTER(_OrParams)]
@@ -66,8 +67,8 @@ def lib():
         L.or_forcing_table.argtypes = [vp, vp, vp, vp, vp, vp]
         L.or_tables.argtypes = [vp, vp, vp]
         L.or_mask.argtypes = [vp, vp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a):
@@ -84,34 +85,35 @@ def rann_sequence(seed: int, n: int):
 class Oracle:
     """One oracle context for a Params (gturbo3d_amd.params.Params)."""
 
-    def __init__(self, p):
+    def __init__(self, p, lib_path: str | None = None):
         self.p = p
+        self._L = lib(lib_path)
         s = _OrParams(p.nx, p.ny, p.nz, p.xlx, p.xly, p.xlz, p.nu, p.alpnu, p.mu, p.alpmu, p.dt,
                       p.omega, p.rtrunc, p.truncate, p.famp, p.k1f, p.k2f, p.variant, p.dt2_mode)
-        self.h = lib().or_create(ctypes.byref(s))
+        self.h = self._L.or_create(ctypes.byref(s))
         if not self.h:
             raise MemoryError("or_create failed")
-        err = lib().or_error(self.h)
+        err = self._L.or_error(self.h)
         if err:
             raise RuntimeError(f"oracle init error {err} (2 = NFORC too small)")
         self.shape = (p.nz, p.ny, p.nx // 2 + 1)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_destroy(self.h)
+            self._L.or_destroy(self.h)
             self.h = None
 
     # ---- tables ----
     @property
     def nf(self) -> int:
-        return lib().or_nf(self.h)
+        return self._L.or_nf(self.h)
 
     def forcing_table(self):
         nf = self.nf
         ikx, iky, ikz = (np.zeros(max(nf, 1), np.int32) for _ in range(3))
         ff = np.zeros(max(nf, 1))
         inp = np.zeros(2)
-        lib().or_forcing_table(self.h, _p(ikx), _p(iky), _p(ikz), _p(ff), _p(inp))
+        self._L.or_forcing_table(self.h, _p(ikx), _p(iky), _p(ikz), _p(ff), _p(inp))
         return ikx[:nf], iky[:nf], ikz[:nf], ff[:nf], inp[0], inp[1]
 
     def tables(self):
@@ -119,7 +121,7 @@ class Oracle:
         nxh = p.nx // 2 + 1
         out = np.zeros(nxh + p.ny + p.nz + 4)
         emk = np.zeros(self.shape)
-        lib().or_tables(self.h, _p(out), _p(emk))
+        self._L.or_tables(self.h, _p(out), _p(emk))
         fkx, fky, fkz = out[:nxh], out[nxh:nxh + p.ny], out[nxh + p.ny:nxh + p.ny + p.nz]
         smax = out[-4:-1]
         return dict(fkx=fkx, fky=fky, fkz=fkz, fkxsmax=smax[0], fkysmax=smax[1], fkzsmax=smax[2],
@@ -127,7 +129,7 @@ class Oracle:
 
     def mask(self):
         m = np.zeros(self.shape, np.uint8)
-        lib().or_mask(self.h, _p(m))
+        self._L.or_mask(self.h, _p(m))
         return m.astype(bool)
 
     # ---- transforms (padded real <-> spectral, in place on a complex array) ----
@@ -136,18 +138,18 @@ class Oracle:
         z = np.zeros(self.shape, np.complex128)
         zr = z.view(np.float64).reshape(self.p.nz, self.p.ny, self.p.nx + 2)
         zr[:, :, :self.p.nx] = phys
-        lib().or_fft_dir(self.h, _p(z))
+        self._L.or_fft_dir(self.h, _p(z))
         return z
 
     def fft_inv(self, spec):
         z = np.ascontiguousarray(spec, dtype=np.complex128).copy()
-        lib().or_fft_inv(self.h, _p(z))
+        self._L.or_fft_inv(self.h, _p(z))
         return z.view(np.float64).reshape(self.p.nz, self.p.ny, self.p.nx + 2)[:, :, :self.p.nx].copy()
 
     # ---- point ops (return new arrays) ----
     def _three(self, fn, a):
         x = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
-        getattr(lib(), fn)(self.h, _p(x[0]), _p(x[1]), _p(x[2]))
+        getattr(self._L, fn)(self.h, _p(x[0]), _p(x[1]), _p(x[2]))
         return x
 
     def curl(self, a):
@@ -161,7 +163,7 @@ class Oracle:
 
     def trunc(self, f):
         x = np.ascontiguousarray(f, dtype=np.complex128).copy()
-        lib().or_trunc(self.h, _p(x))
+        self._L.or_trunc(self.h, _p(x))
         return x
 
     def cross(self, a, b):
@@ -174,7 +176,7 @@ class Oracle:
             return z
         A = [pad(f) for f in a]
         B = [pad(f) for f in b]
-        lib().or_cross(self.h, *[_p(x) for x in A + B])
+        self._L.or_cross(self.h, *[_p(x) for x in A + B])
         return [x[:, :, :p.nx].copy() for x in A]
 
     def nlt(self, a, b):
@@ -182,32 +184,32 @@ class Oracle:
         A = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
         V = [np.zeros(self.shape, np.complex128) for _ in range(3)]
         Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
-        lib().or_nlt(self.h, *[_p(x) for x in A + V + Bf])
+        self._L.or_nlt(self.h, *[_p(x) for x in A + V + Bf])
         return A, V
 
     def step1(self, a, c):
         C = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in c]
         Af = [np.ascontiguousarray(f, dtype=np.complex128) for f in a]
-        lib().or_step1(self.h, *[_p(x) for x in Af + C])
+        self._L.or_step1(self.h, *[_p(x) for x in Af + C])
         return C
 
     def step2(self, a, c):
         A = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in a]
         Cf = [np.ascontiguousarray(f, dtype=np.complex128) for f in c]
-        lib().or_step2(self.h, *[_p(x) for x in A + Cf])
+        self._L.or_step2(self.h, *[_p(x) for x in A + Cf])
         return A
 
     def forcing(self, b, seed: int):
         B = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in b]
         s = ctypes.c_int32(seed)
-        lib().or_forcing(self.h, _p(B[0]), _p(B[1]), _p(B[2]), ctypes.byref(s))
+        self._L.or_forcing(self.h, _p(B[0]), _p(B[1]), _p(B[2]), ctypes.byref(s))
         return B, s.value
 
     def step(self, b, seed: int, nsteps: int = 1):
         B = [np.ascontiguousarray(f, dtype=np.complex128).copy() for f in b]
         W = [np.zeros(self.shape, np.complex128) for _ in range(6)]
         s = ctypes.c_int32(seed)
-        lib().or_step(self.h, *[_p(x) for x in B + W], ctypes.byref(s), nsteps)
+        self._L.or_step(self.h, *[_p(x) for x in B + W], ctypes.byref(s), nsteps)
         return B, s.value
 
     def init_from_velocity(self, u):
@@ -218,14 +220,14 @@ class Oracle:
             zr = z.view(np.float64).reshape(p.nz, p.ny, p.nx + 2)
             zr[:, :, :p.nx] = f
             Z.append(z)
-        lib().or_init_from_velocity(self.h, _p(Z[0]), _p(Z[1]), _p(Z[2]))
+        self._L.or_init_from_velocity(self.h, _p(Z[0]), _p(Z[1]), _p(Z[2]))
         return Z
 
     def velocity(self, b):
         p = self.p
         Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
         U = [np.zeros(self.shape, np.complex128) for _ in range(3)]
-        lib().or_velocity(self.h, *[_p(x) for x in Bf + U])
+        self._L.or_velocity(self.h, *[_p(x) for x in Bf + U])
         return [u.view(np.float64).reshape(p.nz, p.ny, p.nx + 2)[:, :, :p.nx].copy() for u in U]
 
     # ---- diagnostics ----
@@ -233,20 +235,20 @@ class Oracle:
         nb = self.p.nx // 2
         sp, sp2 = np.zeros(nb), np.zeros(nb)
         Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
-        lib().or_spectrum(self.h, *[_p(x) for x in Bf], _p(sp), _p(sp2))
+        self._L.or_spectrum(self.h, *[_p(x) for x in Bf], _p(sp), _p(sp2))
         return sp, sp2
 
     def fluxes(self, b, a):
         nb = self.p.nx // 2
         spf = np.zeros(nb)
         X = [np.ascontiguousarray(f, dtype=np.complex128) for f in list(b) + list(a)]
-        lib().or_fluxes(self.h, *[_p(x) for x in X], _p(spf))
+        self._L.or_fluxes(self.h, *[_p(x) for x in X], _p(spf))
         return spf
 
     def glob(self, b):
         out = np.zeros(18)
         Bf = [np.ascontiguousarray(f, dtype=np.complex128) for f in b]
-        lib().or_glob(self.h, *[_p(x) for x in Bf], _p(out))
+        self._L.or_glob(self.h, *[_p(x) for x in Bf], _p(out))
         return out
 
 
