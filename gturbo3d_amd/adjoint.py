@@ -33,7 +33,12 @@ def to_complex(t: torch.Tensor):
 
 
 class StepRun(torch.autograd.Function):
-    """b_N = run(b_0, amps): nsteps steps on context ``g`` from forcing seed ``seed``."""
+    """b_N = run(b_0, amps): nsteps steps on context ``g`` from forcing seed ``seed``.
+
+    The checkpoints live in the native context, which holds one run at a
+    time: a second run on the same context before this node's backward
+    replaces them, and this node's backward then raises instead of
+    differentiating the other run.  Use one context per live run."""
 
     @staticmethod
     def forward(ctx, b0, amps, g, nsteps, seed, every):
@@ -41,10 +46,14 @@ class StepRun(torch.autograd.Function):
         g.set_shell_amps(amps.detach().to("cpu", torch.float64).numpy())
         ctx.seed_out = g.adjoint_forward(int(nsteps), int(seed), int(every))
         ctx.g = g
+        ctx.run = g.adjoint_run
         return to_real(g.get_state()).to(b0.device)
 
     @staticmethod
     def backward(ctx, grad):
+        if ctx.g.adjoint_run != ctx.run:
+            raise RuntimeError(f"StepRun.backward: the context's checkpoints belong to run {ctx.g.adjoint_run}, "
+                               f"not to this node's run {ctx.run} (one live run per context)")
         gb, ga = ctx.g.adjoint_backward(to_complex(grad))
         dev = grad.device
         return to_real(gb).to(dev), torch.from_numpy(ga.copy()).to(dev), None, None, None, None

@@ -31,8 +31,22 @@ FLAGS = [
 ]
 
 
-def _compile(src: str, obj_dir: str = OBJ, defines=()) -> str:
+def _stale(src: str, out: str) -> bool:
+    """True when the object is older than its source or any header of csrc/ or include/."""
+    if not os.path.exists(out):
+        return True
+    t = os.path.getmtime(out)
+    deps = [os.path.join(CSRC, src)]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    if "gturbo3d.h" in open(deps[0]).read():
+        deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src: str, obj_dir: str = OBJ, defines=(), incremental: bool = False) -> str:
     out = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, out):
+        return out
     cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-c",
            os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -42,12 +56,13 @@ def _compile(src: str, obj_dir: str = OBJ, defines=()) -> str:
     return out
 
 
-def build_lib(verbose: bool = True, out: str = LIB, defines=()) -> str:
-    """Compile and link the library; ``defines`` are extra -D options (tile-shape overrides)."""
+def build_lib(verbose: bool = True, out: str = LIB, defines=(), incremental: bool = False) -> str:
+    """Compile and link the library; ``defines`` are extra -D options (tile-shape overrides).
+    ``incremental`` recompiles only sources older than their object files."""
     obj_dir = OBJ if not defines else os.path.join(OBJ, "_".join(defines).replace("=", ""))
     os.makedirs(obj_dir, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, obj_dir, defines), SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, obj_dir, defines, incremental), SOURCES))
     tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -64,7 +79,7 @@ def build_driver(verbose: bool = True) -> str | None:
     fc = shutil.which("amdflang") or "/opt/rocm/lib/llvm/bin/amdflang"
     src = os.path.join(PKG, "driver", "gturb_driver.f90")
     if not (os.path.exists(fc) and os.path.exists(src)):
-        return None
+        raise RuntimeError(f"cannot build the Fortran driver: {fc} or {src} missing")
     exe = os.path.join(PKG, "driver", "gturb_mi355x")
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
@@ -78,6 +93,6 @@ def build_driver(verbose: bool = True) -> str | None:
 
 
 if __name__ == "__main__":
-    build_lib()
+    build_lib(incremental="--incremental" in sys.argv)
     if "--no-driver" not in sys.argv:
         build_driver()

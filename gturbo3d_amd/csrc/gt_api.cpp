@@ -78,6 +78,9 @@ struct gt_ctx {
   int zchunk = 0;   // z planes per chunk of passes B, C (and D) (0: all at once)
   bool zchunk_d = true;  // pass D runs in the chunks too (else once, after them)
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
+  // bumped whenever something an ensemble's captured graphs bake in changes:
+  // the pruned/dense mode (extents of every launch), the tile widths, the z chunk
+  unsigned gen = 0;
   int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
   // batches: nmem members with their own omega (and forcing seeds)
@@ -606,13 +609,18 @@ int ensure_force_cap(gt_ctx* c, size_t n) {
 
 // Set pruned mode and make bh vanish outside the kept extents.
 int set_mode(gt_ctx* c, bool pruned) {
+  if (c->pruned != pruned) c->gen++;
   c->pruned = pruned;
   HIPCHK(hipMemsetAsync(c->bh, 0, sizeof(cx<double>) * 3 * c->fs * c->nmem, c->stream));
   return GT_OK;
 }
 
+// Every entry point on a context starts here: a context bound to an
+// ensemble first waits for the ensemble's stream, which runs on the same
+// device fields.
 int check_ctx(gt_ctx* c) {
   if (!c) return fail(GT_EINVAL, "null context");
+  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
   return GT_OK;
 }
 
@@ -971,9 +979,16 @@ int run_diag(gt_ctx* c, int member, const void* nl, std::vector<double>& host) {
 }
 }  // namespace
 
+extern "C" int gt_check_finite(const double* v, int n) {
+  if (!v && n > 0) return fail(GT_EINVAL, "gt_check_finite: null values");
+  for (int q = 0; q < n; q++)
+    if (!std::isfinite(v[q]))
+      return fail(GT_ENONFINITE, "non-finite value " + std::to_string(v[q]) + " at index " + std::to_string(q));
+  return GT_OK;
+}
+
 static int spectra_m(gt_ctx* c, int member, double* sp, double* sp2d, double* hel, double* glob) {
   if (int rc = check_ctx(c)) return rc;
-  if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
   if (member < 0 || member >= c->nmem) return fail(GT_EINVAL, "spectra: member out of range");
   std::vector<double> h;
   if (int rc = run_diag(c, member, nullptr, h)) return rc;
@@ -985,6 +1000,10 @@ static int spectra_m(gt_ctx* c, int member, double* sp, double* sp2d, double* he
   }
   if (glob)
     for (int q = 0; q < 18; q++) glob[q] = h[4 * nbin + q];
+  // the glob sums cover every mode with k != 0: a NaN or an infinity anywhere
+  // in the state makes one of them non-finite
+  if (gt_check_finite(h.data() + 4 * nbin, 18) != GT_OK)
+    return fail(GT_ENONFINITE, "spectra: the state is not finite (" + g_err + ")");
   return GT_OK;
 }
 
@@ -1151,6 +1170,7 @@ int gt_set_tiles(gt_ctx* c, int tz, int tze, int ty) {
   c->tz = (tz == pencil_T(c->nz)) ? 0 : tz;
   c->tze = (tze == pencil_T(c->nz)) ? 0 : tze;
   c->ty = (ty == pencil_T(c->ny)) ? 0 : ty;
+  c->gen++;
   return GT_OK;
 }
 
@@ -1159,6 +1179,7 @@ int gt_set_zchunk(gt_ctx* c, int planes, int with_d) {
   if (planes < 0) return fail(GT_EINVAL, "gt_set_zchunk: planes must be >= 0");
   c->zchunk = (planes >= c->nz) ? 0 : planes;
   c->zchunk_d = with_d != 0;
+  c->gen++;
   return GT_OK;
 }
 
@@ -1271,6 +1292,7 @@ int gt_autotune(gt_ctx* c, int nrep) {
     if (e0) c->pool.push_back(e0);
     if (e1) c->pool.push_back(e1);
   }
+  c->gen++;  // widths and z chunk may have changed
   if (rc != GT_OK) return rc;
   return set_mode(c, c->pruned);  // bh vanishes outside the kept extents again
 }
@@ -1311,6 +1333,12 @@ struct gt_ensemble {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<FEnt*> hbuf[2];  // per batch
+  // per batch: the device forcing buffer the graphs copy into (the ensemble's
+  // own, so that nothing a batch call reallocates is baked into a graph)
+  std::vector<FEnt*> dbuf;
+  // per batch: the generation of the context when the graphs were captured;
+  // a batch whose generation moved on gets the graphs recaptured
+  std::vector<unsigned> gen;
   hipEvent_t done[2] = {nullptr, nullptr};
   bool used[2] = {false, false};
   int next = 0;
@@ -1330,21 +1358,40 @@ void free_ensemble(gt_ensemble* e) {
       if (h) hipHostFree(h);
     if (e->done[g]) hipEventDestroy(e->done[g]);
   }
+  for (FEnt* d : e->dbuf)
+    if (d) hipFree(d);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
 }
 
 int ensemble_capture(gt_ensemble* e, int g) {
-  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  if (e->exec[g]) {
+    hipGraphExecDestroy(e->exec[g]);
+    e->exec[g] = nullptr;
+  }
+  if (e->graph[g]) {
+    hipGraphDestroy(e->graph[g]);
+    e->graph[g] = nullptr;
+  }
+  // no timing events inside a captured graph
+  std::vector<char> prof(e->batches.size());
+  for (size_t q = 0; q < e->batches.size(); q++) {
+    prof[q] = e->batches[q]->prof;
+    e->batches[q]->prof = false;
+  }
   int rc = GT_OK;
+  hipError_t berr = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+  if (berr != hipSuccess) rc = fail(GT_EHIP, std::string("ensemble capture: ") + hipGetErrorString(berr));
   for (size_t q = 0; q < e->batches.size() && rc == GT_OK; q++) {
     gt_ctx* c = e->batches[q];
     const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)e->chunk;
-    if (need > 0 && hipMemcpyAsync(c->dforce, e->hbuf[g][q], sizeof(FEnt) * need, hipMemcpyHostToDevice, e->stream) !=
-                        hipSuccess)
+    if (need > 0 && hipMemcpyAsync(e->dbuf[q], e->hbuf[g][q], sizeof(FEnt) * need, hipMemcpyHostToDevice,
+                                   e->stream) != hipSuccess)
       rc = fail(GT_EHIP, "ensemble capture: forcing copy");
-    if (rc == GT_OK) rc = enqueue_steps(c, e->stream, e->chunk, c->dforce);
+    if (rc == GT_OK) rc = enqueue_steps(c, e->stream, e->chunk, e->dbuf[q]);
   }
+  for (size_t q = 0; q < e->batches.size(); q++) e->batches[q]->prof = prof[q] != 0;
+  if (berr != hipSuccess) return rc;
   hipGraph_t graph = nullptr;
   hipError_t err = hipStreamEndCapture(e->stream, &graph);
   if (rc != GT_OK) {
@@ -1354,6 +1401,19 @@ int ensemble_capture(gt_ensemble* e, int g) {
   if (err != hipSuccess) return fail(GT_EHIP, std::string("ensemble capture: ") + hipGetErrorString(err));
   e->graph[g] = graph;
   HIPCHK(hipGraphInstantiate(&e->exec[g], graph, nullptr, nullptr, 0));
+  return GT_OK;
+}
+
+// Recapture both graphs when a batch changed what they bake in (its mode,
+// tile widths or z chunk) since the last capture.
+int ensemble_refresh(gt_ensemble* e) {
+  bool stale = false;
+  for (size_t q = 0; q < e->batches.size(); q++) stale = stale || e->batches[q]->gen != e->gen[q];
+  if (!stale) return GT_OK;
+  HIPCHK(hipStreamSynchronize(e->stream));  // no replay of the old graphs is in flight
+  for (int g = 0; g < 2; g++)
+    if (int rc = ensemble_capture(e, g)) return rc;
+  for (size_t q = 0; q < e->batches.size(); q++) e->gen[q] = e->batches[q]->gen;
   return GT_OK;
 }
 
@@ -1374,26 +1434,36 @@ int gt_ensemble_create(gt_ctx** batches, int nbatches, int chunk, gt_ensemble** 
     e->batches.push_back(batches[q]);
     e->seeds.emplace_back(batches[q] ? batches[q]->nmem : 0, 0);
   }
+  for (int q = 0; q < nbatches && rc == GT_OK; q++)
+    for (int r = 0; r < q; r++)
+      if (batches[q] == batches[r]) rc = fail(GT_EINVAL, "gt_ensemble_create: a batch is listed twice");
+  for (int q = 0; q < nbatches && rc == GT_OK; q++)
+    if (batches[q]->ext_stream) rc = fail(GT_EINVAL, "gt_ensemble_create: a batch already belongs to an ensemble");
   if (rc == GT_OK && hipStreamCreate(&e->stream) != hipSuccess) rc = fail(GT_EHIP, "gt_ensemble_create: stream");
+  for (gt_ctx* c : e->batches) {
+    if (rc != GT_OK) break;
+    const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)chunk;
+    FEnt* d = nullptr;
+    if (need > 0 && hipMalloc((void**)&d, sizeof(FEnt) * need) != hipSuccess)
+      rc = fail(GT_ENOMEM, "gt_ensemble_create: device forcing buffer");
+    e->dbuf.push_back(d);
+  }
   for (int g = 0; g < 2 && rc == GT_OK; g++) {
     if (hipEventCreateWithFlags(&e->done[g], hipEventDisableTiming) != hipSuccess)
       rc = fail(GT_EHIP, "gt_ensemble_create: event");
     for (gt_ctx* c : e->batches) {
       if (rc != GT_OK) break;
-      c->prof = false;  // no timing events inside a captured graph
       const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)chunk;
       FEnt* h = nullptr;
-      if (need > 0) {
-        if (int r = ensure_force_cap(c, need)) rc = r;
-        else if (hipHostMalloc((void**)&h, sizeof(FEnt) * need, hipHostMallocDefault) != hipSuccess)
-          rc = fail(GT_ENOMEM, "gt_ensemble_create: pinned forcing buffer");
-      }
+      if (need > 0 && hipHostMalloc((void**)&h, sizeof(FEnt) * need, hipHostMallocDefault) != hipSuccess)
+        rc = fail(GT_ENOMEM, "gt_ensemble_create: pinned forcing buffer");
       e->hbuf[g].push_back(h);
     }
   }
   for (gt_ctx* c : e->batches) {
     if (rc == GT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(GT_EHIP, "gt_ensemble_create: sync");
     if (rc == GT_OK) c->ext_stream = e->stream;
+    e->gen.push_back(c ? c->gen : 0);
   }
   for (int g = 0; g < 2 && rc == GT_OK; g++) rc = ensemble_capture(e, g);
   if (rc != GT_OK) {
@@ -1429,6 +1499,10 @@ int gt_ensemble_get_seeds(gt_ensemble* e, int batch, int32_t* seeds) {
 int gt_ensemble_step(gt_ensemble* e, int nsteps) {
   if (!e) return fail(GT_EINVAL, "gt_ensemble_step: null ensemble");
   if (nsteps < 0) return fail(GT_EINVAL, "gt_ensemble_step: negative step count");
+  // work a batch call queued on its own stream (a set_state copy, a
+  // diagnostic) touches the same device fields: it finishes first
+  for (gt_ctx* c : e->batches) HIPCHK(hipStreamSynchronize(c->stream));
+  if (int rc = ensemble_refresh(e)) return rc;
   int left = nsteps;
   while (left >= e->chunk) {
     const int g = e->next;
@@ -1442,21 +1516,20 @@ int gt_ensemble_step(gt_ensemble* e, int nsteps) {
     left -= e->chunk;
   }
   if (left > 0) {
-    HIPCHK(hipStreamSynchronize(e->stream));  // the device forcing buffers are shared with the graphs
+    // the device forcing buffers are shared with the graphs, and the pinned
+    // buffers of graph `next` are free once the stream is idle
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int g = e->next;
     for (size_t q = 0; q < e->batches.size(); q++) {
       gt_ctx* c = e->batches[q];
       const size_t need = entries_per_step(c) * (size_t)c->nmem * (size_t)left;
       if (need > 0) {
-        if (c->hforce_pending) {
-          HIPCHK(hipEventSynchronize(c->hforce_done));
-          c->hforce_pending = false;
-        }
-        fill_forcing(c, left, e->seeds[q].data(), c->hforce);
-        HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * need, hipMemcpyHostToDevice, e->stream));
+        fill_forcing(c, left, e->seeds[q].data(), e->hbuf[g][q]);
+        HIPCHK(hipMemcpyAsync(e->dbuf[q], e->hbuf[g][q], sizeof(FEnt) * need, hipMemcpyHostToDevice, e->stream));
       } else {
         fill_forcing(c, left, e->seeds[q].data(), nullptr);
       }
-      if (int rc = enqueue_steps(c, e->stream, left, c->dforce)) return rc;
+      if (int rc = enqueue_steps(c, e->stream, left, e->dbuf[q])) return rc;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
   }

@@ -4,9 +4,33 @@
 ! (src/main.f90): params.dat (src/inieuler.f90:14-20), curframe.dat
 ! (src/main.f90:37-39, 213-215), files/seed.NNN (src/main.f90:42-45,
 ! 207-210), restart velocity fields/u{x,y,z}.NNN as float32 unformatted
-! records (src/read.f90:1-66, src/write.f90:1-54) and the diagnostics
-! cadence imix (src/main.f90:134-138, 153-161).  The time step itself is the
-! library's: gt_step runs the loop body of src/main.f90:124-150 on the GPU.
+! records (src/read.f90:1-66, src/write.f90:1-54), and the time loop of
+! src/main.f90:121-185 with its outputs:
+!   every imix steps, of the state at time t before the step (:133-138):
+!     files/fluxes.NNN  (src/diagnostics.f90:62-103)
+!     files/spectra.NNN (src/diagnostics.f90:5-56)
+!   every imix steps, after the step at t+dt (:152-161):
+!     files/globalE.NNN, files/globalZ.NNN, 11 columns each
+!     (src/diagnostics.f90:114-199), and the 't = , Eb = , E =' line
+!   every iout steps except the last (:163-184): fields/u?.NNN.JJJ (writef2,
+!     src/write.f90:56-109)
+! The time step itself is the library's: gt_step runs the loop body of
+! src/main.f90:124-150 on the GPU; the diagnostics are the library's device
+! reductions (gt_fluxes, gt_spectra).  Numbers are written as es24.16 doubles
+! (the reference's 'g' edit descriptor without a width is compiler-specific).
+!
+! A diagnostics step whose energy or dissipation sum is not finite ends the
+! run: the library returns GT_ENONFINITE, the driver reports the step and
+! time on stderr and stops with exit status 3, before writing anything of
+! that step.
+!
+! Records of more than MAXSUB = 2147483639 bytes (a float32 field of 1024^3
+! and above is 4 GiB, whose 4-byte record marker would wrap) are written and
+! read as subrecords in gfortran's documented layout: every subrecord has a
+! leading and a trailing 4-byte length, the leading one negative when
+! another subrecord follows and the trailing one negative when one precedes.
+! Smaller records are one plain record, byte for byte what the reference
+! writes.  The environment variable GT_MAX_SUBRECORD lowers the limit (tests).
 !
 ! The grid is a run-time argument (the reference fixes it at compile time in
 ! src/modules.f90:4):
@@ -73,6 +97,15 @@ module gturbo3d_api
       real(c_double), intent(out) :: inpe, inpz
       integer(c_int), value :: cap
     end function
+    integer(c_int) function gt_spectra(ctx, sp, sp2d, hel, glob) bind(C, name="gt_spectra")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, sp, sp2d, hel, glob
+    end function
+    integer(c_int) function gt_fluxes(ctx, spf) bind(C, name="gt_fluxes")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: spf(*)
+    end function
     integer(c_int) function gt_sync(ctx) bind(C, name="gt_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -81,6 +114,8 @@ module gturbo3d_api
       import :: c_ptr
     end function
   end interface
+
+  integer(c_int), parameter :: GT_ENONFINITE = 6
 
 contains
 
@@ -105,39 +140,100 @@ end module gturbo3d_api
 module driver_io
   use iso_c_binding
   implicit none
+  integer(8), parameter :: MAXSUB = 2147483639_8  ! gfortran's largest subrecord (4-byte markers)
 contains
 
-  ! float32 physical field, one unformatted record (src/read.f90:10-22)
+  ! the subrecord limit in bytes: MAXSUB, or GT_MAX_SUBRECORD when it is smaller
+  integer(8) function max_subrecord()
+    character(len=32) :: v
+    integer :: st
+    integer(8) :: lim
+    max_subrecord = MAXSUB
+    call get_environment_variable('GT_MAX_SUBRECORD', v, status=st)
+    if (st == 0) then
+      read(v, *) lim
+      if (lim >= 1 .and. lim < MAXSUB) max_subrecord = lim
+    end if
+  end function max_subrecord
+
+  ! float32 physical field, one logical unformatted record (src/read.f90:10-22)
   subroutine read_field(path, nx, ny, nz, u)
     character(len=*), intent(in) :: path
     integer, intent(in) :: nx, ny, nz
     real(c_double), intent(out) :: u(nx, ny, nz)
-    real(4), allocatable :: w(:, :, :)
+    real(4), allocatable, target :: w(:)
+    integer(1), pointer :: wb(:)
+    integer(8) :: nbytes, pos, slen
+    integer(4) :: lead, trail
     integer :: lu, ios
-    allocate(w(nx, ny, nz))
-    open(newunit=lu, file=path, form='unformatted', action='read', status='old', iostat=ios)
+    nbytes = 4_8 * int(nx, 8) * int(ny, 8) * int(nz, 8)
+    allocate(w(nbytes / 4))
+    call c_f_pointer(c_loc(w), wb, [nbytes])
+    open(newunit=lu, file=path, access='stream', form='unformatted', action='read', status='old', iostat=ios)
     if (ios /= 0) then
       write(0, *) 'cannot open ', trim(path)
       stop 2
     end if
-    read(lu, iostat=ios) w
-    if (ios /= 0) then
+    pos = 0
+    do
+      read(lu, iostat=ios) lead
+      if (ios /= 0) exit
+      slen = abs(int(lead, 8))
+      if (pos + slen > nbytes) then
+        ios = 1
+        exit
+      end if
+      if (slen > 0) read(lu, iostat=ios) wb(pos + 1:pos + slen)
+      if (ios /= 0) exit
+      read(lu, iostat=ios) trail
+      if (ios /= 0 .or. abs(int(trail, 8)) /= slen) then
+        ios = 1
+        exit
+      end if
+      pos = pos + slen
+      if (lead >= 0) exit
+    end do
+    close(lu)
+    if (ios /= 0 .or. pos /= nbytes) then
       write(0, *) 'short or corrupt field file ', trim(path)
       stop 2
     end if
-    close(lu)
-    u = dble(w)
+    u = reshape(dble(w), [nx, ny, nz])
     deallocate(w)
   end subroutine read_field
 
+  ! float32 physical field as one logical record (src/write.f90:12-22), split
+  ! into subrecords above max_subrecord() bytes
   subroutine write_field(path, nx, ny, nz, u)
     character(len=*), intent(in) :: path
     integer, intent(in) :: nx, ny, nz
     real(c_double), intent(in) :: u(nx, ny, nz)
+    real(4), allocatable, target :: w(:)
+    integer(1), pointer :: wb(:)
+    integer(8) :: nbytes, pos, slen, lim
+    integer(4) :: lead, trail
     integer :: lu
-    open(newunit=lu, file=path, form='unformatted', status='replace')
-    write(lu) real(u, 4)
+    nbytes = 4_8 * int(nx, 8) * int(ny, 8) * int(nz, 8)
+    allocate(w(nbytes / 4))
+    w = reshape(real(u, 4), [nbytes / 4])
+    call c_f_pointer(c_loc(w), wb, [nbytes])
+    lim = max_subrecord()
+    open(newunit=lu, file=path, access='stream', form='unformatted', status='replace')
+    pos = 0
+    do
+      slen = min(lim, nbytes - pos)
+      lead = int(slen, 4)
+      trail = int(slen, 4)
+      if (pos + slen < nbytes) lead = -lead  ! another subrecord follows
+      if (pos > 0) trail = -trail            ! a subrecord precedes
+      write(lu) lead
+      if (slen > 0) write(lu) wb(pos + 1:pos + slen)
+      write(lu) trail
+      pos = pos + slen
+      if (pos >= nbytes) exit
+    end do
     close(lu)
+    deallocate(w)
   end subroutine write_field
 
 end module driver_io
@@ -150,16 +246,14 @@ program gturb_mi355x
 
   type(gt_params) :: p
   type(c_ptr) :: ctx
-  integer :: nx, ny, nz, nxh, npas, iout, imix, ifr, ipas, nchunk, nbin, lu, i, j, k, ib
+  integer :: nx, ny, nz, nxh, npas, iout, imix, ifr, jfr, ipas, nxt, nrun, nbin, lu, ib, rc
   integer(c_int32_t) :: seed
   integer(c_int) :: nf
-  real(c_double) :: t, inpe, inpz, tmp(9), fac, kk, k2, dkx, dky, dkz, scra, e1, e2
-  real(c_double), allocatable :: ux(:, :, :), uy(:, :, :), uz(:, :, :), sp(:), sp2d(:)
-  real(c_double), allocatable :: fkx(:), fky(:), fkz(:)
-  complex(c_double_complex), allocatable :: bx(:, :, :), by(:, :, :), bz(:, :, :)
+  real(c_double) :: t, inpe, inpz
+  real(c_double), allocatable :: ux(:, :, :), uy(:, :, :), uz(:, :, :)
+  real(c_double), allocatable, target :: sp(:), sp2d(:), spf(:), glob(:)
   character(len=64) :: arg, nome
   integer(8) :: c0, c1, crate
-  real(c_double), parameter :: pi = 3.14159265358979323846d0
 
   call system_clock(c0, crate)
   if (command_argument_count() < 3) then
@@ -209,8 +303,7 @@ program gturb_mi355x
   write(6, '(a,i0,a,es14.6)') ' forced modes ', nf, ', mean energy input ', inpe
 
   allocate(ux(nx, ny, nz), uy(nx, ny, nz), uz(nx, ny, nz))
-  allocate(bx(nxh, ny, nz), by(nxh, ny, nz), bz(nxh, ny, nz))
-  allocate(sp(nbin), sp2d(nbin), fkx(nxh), fky(ny), fkz(nz))
+  allocate(sp(nbin), sp2d(nbin), spf(nbin), glob(18))
 
   ! restart velocity -> b (src/main.f90:91-106)
   write(nome, "('./fields/ux.',i3.3)") ifr; call read_field(nome, nx, ny, nz, ux)
@@ -218,61 +311,71 @@ program gturb_mi355x
   write(nome, "('./fields/uz.',i3.3)") ifr; call read_field(nome, nx, ny, nz, uz)
   call check(gt_init_velocity(ctx, ux, uy, uz), 'gt_init_velocity')
 
-  ! wavenumbers for the host diagnostics (src/defk.f90:24-43)
-  dkx = 2.0d0 * pi / p%xlx; dky = 2.0d0 * pi / p%xly; dkz = 2.0d0 * pi / p%xlz
-  do i = 1, nxh
-    fkx(i) = dble(i - 1) * dkx
-  end do
-  do j = 1, ny
-    fky(j) = dble(merge(j - 1, j - ny - 1, j <= ny / 2 + 1)) * dky
-  end do
-  do k = 1, nz
-    fkz(k) = dble(merge(k - 1, k - nz - 1, k <= nz / 2 + 1)) * dkz
-  end do
-
+  ! output files (src/main.f90:50-61)
   write(nome, "('./files/globalE.',i3.3)") ifr
   open(unit=18, file=nome, form='formatted', status='unknown', position='append')
+  write(nome, "('./files/globalZ.',i3.3)") ifr
+  open(unit=19, file=nome, form='formatted', status='unknown', position='append')
   write(nome, "('./files/spectra.',i3.3)") ifr
   open(unit=20, file=nome, form='formatted', status='unknown', position='append')
+  write(nome, "('./files/fluxes.',i3.3)") ifr
+  open(unit=30, file=nome, form='formatted', status='unknown', position='append')
 
+  ! time loop (src/main.f90:121-185): the steps between two output steps run
+  ! as one gt_step call; an output step runs alone, with its diagnostics of
+  ! time t before it and of t+dt after it
+  jfr = 1
   ipas = 0
   do while (ipas < npas)
-    nchunk = min(imix - mod(ipas, imix), npas - ipas)
-    call check(gt_step(ctx, nchunk, seed), 'gt_step')
-    ipas = ipas + nchunk
-    t = t + dble(nchunk) * p%dt
-    if (mod(ipas, imix) == 0) then
-      call check(gt_get_state(ctx, bx, by, bz), 'gt_get_state')
-      ! energy spectrum sum fac k^2 |b|^2 (src/diagnostics.f90:22-35) and glob energy (:131-153)
-      sp = 0.d0; sp2d = 0.d0; tmp = 0.d0
-      do k = 1, nz
-        do j = 1, ny
-          do i = 1, nxh
-            fac = 1.d0
-            if (i == 1 .or. i == nxh) fac = 0.5d0
-            k2 = fkx(i)**2 + fky(j)**2 + fkz(k)**2
-            kk = sqrt(k2)
-            scra = abs(bx(i, j, k))**2 + abs(by(i, j, k))**2 + abs(bz(i, j, k))**2
-            ib = int(kk) + 1
-            if (ib > 0 .and. ib < nbin) sp(ib) = sp(ib) + kk * kk * fac * scra
-            if (k == 1 .and. ib > 0 .and. ib < nbin) sp2d(ib) = sp2d(ib) + kk * kk * fac * abs(bz(i, j, k))**2
-            if (k2 > 0.d0) then
-              tmp(1) = tmp(1) + fac * k2 * scra
-              tmp(4) = tmp(4) + 2.d0 * p%nu * fac * k2 * scra * k2**p%alpnu
-              tmp(7) = tmp(7) + 2.d0 * p%mu * fac * k2 * scra * k2**p%alpmu
-            end if
-          end do
-        end do
-      end do
-      e1 = tmp(1)
-      e2 = tmp(4) + tmp(7)
-      write(18, '(4es24.16)') t, e1, e2, inpe
-      call flush(18)
+    nxt = npas
+    if (imix > 0) nxt = min(nxt, (ipas / imix + 1) * imix)
+    if (iout > 0) nxt = min(nxt, (ipas / iout + 1) * iout)
+    nrun = nxt - 1 - ipas
+    if (nrun > 0) then
+      call check(gt_step(ctx, nrun, seed), 'gt_step')
+      ipas = ipas + nrun
+      t = t + dble(nrun) * p%dt
+    end if
+    ipas = ipas + 1
+    if (is_every(ipas, imix)) then
+      ! spectrum and flux at time t, before the step (src/main.f90:133-138)
+      rc = gt_spectra(ctx, c_loc(sp), c_loc(sp2d), c_null_ptr, c_null_ptr)
+      if (rc == GT_ENONFINITE) call nonfinite(ipas, t)
+      call check(rc, 'gt_spectra')
+      call check(gt_fluxes(ctx, spf), 'gt_fluxes')
       do ib = 1, nbin
-        write(20, '(3es24.16)') dble(ib - 1), sp(ib), sp2d(ib)
+        write(30, '(i8,es24.16)') ib, spf(ib)
+      end do
+      call flush(30)
+      do ib = 1, nbin
+        write(20, '(i8,2es24.16)') ib - 1, sp(ib), sp2d(ib)
       end do
       call flush(20)
-      write(6, '(a,es14.6,a,es12.4,a,es14.6)') ' t = ', t, ', Eb = ', e2 / inpe, ', E = ', e1
+    end if
+    call check(gt_step(ctx, 1, seed), 'gt_step')
+    t = t + p%dt
+    if (is_every(ipas, imix)) then
+      ! global quantities at t+dt (src/main.f90:152-161, src/diagnostics.f90:126-196)
+      rc = gt_spectra(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(glob))
+      if (rc == GT_ENONFINITE) call nonfinite(ipas, t)
+      call check(rc, 'gt_spectra')
+      write(18, '(11es24.16)') t, glob(1:9), inpe
+      call flush(18)
+      write(6, '(a,es14.6,a,es14.6,a,es14.6)') ' t = ', t, ', Eb = ', sum(glob(4:9)) / inpe, &
+        ', E = ', sum(glob(1:3))
+      call flush(6)
+      write(19, '(11es24.16)') t, glob(10:18), inpz
+      call flush(19)
+    end if
+    if (is_every(ipas, iout) .and. ipas /= npas) then
+      ! velocity snapshot (src/main.f90:163-184, writef2)
+      call check(gt_velocity(ctx, ux, uy, uz), 'gt_velocity')
+      write(nome, "('./fields/ux.',i3.3,'.',i3.3)") ifr, jfr; call write_field(nome, nx, ny, nz, ux)
+      write(nome, "('./fields/uy.',i3.3,'.',i3.3)") ifr, jfr; call write_field(nome, nx, ny, nz, uy)
+      write(nome, "('./fields/uz.',i3.3,'.',i3.3)") ifr, jfr; call write_field(nome, nx, ny, nz, uz)
+      jfr = jfr + 1
+      write(6, '(a,i0)') ' Saved v field at timestep = ', ipas
+      call flush(6)
     end if
   end do
 
@@ -290,9 +393,27 @@ program gturb_mi355x
   write(lu, *) ifr, t
   close(lu)
   close(18)
+  close(19)
   close(20)
+  close(30)
   call check(gt_destroy(ctx), 'gt_destroy')
 
   call system_clock(c1)
   write(6, '(a,es14.6)') ' Mean time per iteration (s) = ', dble(c1 - c0) / dble(crate) / dble(max(npas, 1))
+
+contains
+
+  logical function is_every(n, every)
+    integer, intent(in) :: n, every
+    is_every = every > 0
+    if (is_every) is_every = mod(n, every) == 0
+  end function is_every
+
+  subroutine nonfinite(n, tt)
+    integer, intent(in) :: n
+    real(c_double), intent(in) :: tt
+    write(0, '(a,i0,a,es14.6,a)') ' non-finite state at step ', n, ' (t = ', tt, '): run stopped'
+    call flush(0)
+    stop 3
+  end subroutine nonfinite
 end program gturb_mi355x

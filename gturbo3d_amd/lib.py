@@ -28,7 +28,10 @@ EXPORTS = [
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
     "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
+    "gt_check_finite",
 ]
+
+GT_ENONFINITE = 6
 
 
 class GtParams(ctypes.Structure):
@@ -44,7 +47,16 @@ class GtParams(ctypes.Structure):
 
 
 class GturboError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+def check_finite(v) -> None:
+    """Raise GturboError (code GT_ENONFINITE) unless every value is finite: the
+    host check gt_spectra applies to its glob sums (no device work)."""
+    a = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    _check(load().gt_check_finite(_ptr(a), int(a.size)), "gt_check_finite")
 
 
 _lib = None
@@ -100,6 +112,7 @@ def load():
     L.gt_set_shell_amps.argtypes = [vp, vp, ip]
     L.gt_adjoint_forward.argtypes = [vp, ip, i32p, ip]
     L.gt_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ip]
+    L.gt_check_finite.argtypes = [vp, ip]
     L.gt_last_error.restype = ctypes.c_char_p
     L.gt_last_error.argtypes = []
     _lib = L
@@ -109,7 +122,7 @@ def load():
 def _check(rc: int, what: str) -> None:
     if rc != 0:
         msg = load().gt_last_error().decode(errors="replace")
-        raise GturboError(f"{what} failed (code {rc}): {msg}")
+        raise GturboError(f"{what} failed (code {rc}): {msg}", rc)
 
 
 def _ptr(a: np.ndarray):
@@ -292,8 +305,11 @@ class Context:
 
     def adjoint_forward(self, nsteps: int, seed: int, every: int) -> int:
         """nsteps steps kept for the adjoint (a checkpoint every `every` steps);
-        returns the advanced seed."""
+        returns the advanced seed.  The context keeps the checkpoints of its
+        last forward run only: ``adjoint_run`` counts the runs, so that a
+        holder of an older run can tell its checkpoints were replaced."""
         s = ctypes.c_int32(seed)
+        self.adjoint_run = getattr(self, "adjoint_run", 0) + 1
         _check(load().gt_adjoint_forward(self.h, int(nsteps), ctypes.byref(s), int(every)), "gt_adjoint_forward")
         return s.value
 

@@ -40,6 +40,7 @@ extern "C" {
 #define GT_ENOMEM 3   /* device or host allocation failed */
 #define GT_ENFORC 4   /* forced-mode table overflow ('NFORC too small !', src/iniforcing_3d3c.f90:33) */
 #define GT_ESIZE 5    /* grid size not supported by the compiled kernels */
+#define GT_ENONFINITE 6 /* the state holds a NaN or an infinity (its energy sums are not finite) */
 
 #define GT_FORCING_3D3C 3
 #define GT_FORCING_2D2C 2
@@ -98,8 +99,17 @@ int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double
  *   shell, and the kz = 0 plane's |k|^2 |bz|^2; hel is sum fac Re(u*.w) with
  *   u = curl b, w = curl u).  glob: 18 doubles, the nine sums of glob
  *   (src/diagnostics.f90:126-153) for u followed by the nine for w
- *   (:164-191).  Any output pointer may be NULL. */
+ *   (:164-191).  Any output pointer may be NULL.
+ * Returns GT_ENONFINITE, with every output written, when one of the 18 glob
+ * sums is not finite: they cover every mode with k != 0, so the state holds
+ * a NaN or an infinity.  (The reference has no such check; it is the
+ * divergence flag of a run, at the cadence the caller asks for diagnostics.) */
 int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob);
+
+/* Host only (no device work): GT_OK when the n values are all finite, else
+ * GT_ENONFINITE with gt_last_error() naming the first one that is not.  It is
+ * the test gt_spectra applies to its glob sums. */
+int gt_check_finite(const double* v, int n);
 
 /* Energy flux spectrum of fluxes (src/diagnostics.f90:62-103): spf[i-1] =
  * sum over shells q >= i of fac*|k|^2 Re(b.conj N), N = nlt(b,u,b), for

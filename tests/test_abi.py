@@ -6,8 +6,11 @@ import re
 
 import pytest
 
-from gturbo3d_amd import lib
+import numpy as np
+
+from gturbo3d_amd import initial, lib
 from gturbo3d_amd.params import Params, read_params_dat, write_params_dat
+import oracle_lib as ol
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -69,3 +72,19 @@ def test_reads_the_reference_params_dat_layout(tmp_path):
     assert q.nu == 1e-17 and q.alpnu == 4.0 and q.mu == 0.6 and q.alpmu == -1.0
     assert q.npas == 10000 and q.imix == 1000 and q.truncate == 1
     assert q.famp == 0.1 and q.k1f == 15.0 and q.k2f == 17.0
+
+
+def test_non_finite_state_check_on_the_host():
+    """The host check behind gt_spectra's GT_ENONFINITE status, fed the glob
+    sums (src/diagnostics.f90:126-191) of fields with a NaN or an infinity."""
+    p = Params(nx=16, ny=16, nz=16)
+    o = ol.Oracle(p)
+    b = initial.initial_state(p, seed=1)
+    lib.check_finite(o.glob(b))
+    for bad in (np.nan, np.inf, -np.inf):
+        c = [f.copy() for f in b]
+        c[1][2, 3, 4] = bad
+        with pytest.raises(lib.GturboError) as e:
+            lib.check_finite(o.glob(c))
+        assert e.value.code == lib.GT_ENONFINITE
+        assert b"non-finite value" in lib.load().gt_last_error()

@@ -168,6 +168,26 @@ def test_autograd_function_and_reuse(require_gpu):
             g.adjoint_backward(lam)
 
 
+def test_autograd_second_run_on_one_context_is_refused(require_gpu):
+    """Two live runs on one context: the first node's checkpoints were replaced,
+    so its backward raises instead of differentiating the second run."""
+    p = Params(nx=32, ny=32, nz=32, nu=2e-3, dt=5e-3, omega=2.0, famp=0.1, k1f=2.0, k2f=3.0)
+    say("two autograd runs on one context")
+    b0 = initial.initial_state(p, seed=2)
+    wt = torch.from_numpy(adjoint.band_weights(p, 4.0, 8.0))
+    with lib.Context(p) as g:
+        nsh = len(g.forcing_shells())
+        amps = torch.full((nsh,), p.famp, dtype=torch.float64, requires_grad=True)
+        x1 = adjoint.to_real(b0).requires_grad_(True)
+        x2 = adjoint.to_real(b0).requires_grad_(True)
+        l1 = adjoint.band_energy(adjoint.run(g, x1, amps, 3, 11, every=2), wt)
+        l2 = adjoint.band_energy(adjoint.run(g, x2, amps, 4, 12, every=2), wt)
+        with pytest.raises(RuntimeError, match="one live run per context"):
+            l1.backward()
+        l2.backward()  # the last run is the one the context holds
+        assert x2.grad is not None and torch.isfinite(x2.grad).all()
+
+
 def test_adjoint_rejects_unsupported_contexts(require_gpu):
     p = Params(nx=32, ny=32, nz=32, precision=PREC_FP32_BULK)
     with lib.Context(p) as g:

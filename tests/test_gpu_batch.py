@@ -132,3 +132,67 @@ def test_tile_widths_do_not_change_results(require_gpu, n):
     for key, st in results.items():
         for x, y in zip(st, ref):
             assert np.array_equal(x, y), key
+
+
+def test_ensemble_follows_batch_changes_after_capture(require_gpu):
+    """What the captured graphs bake in can change after gt_ensemble_create: a
+    member set to an untruncated state switches its batch to the dense path, a
+    batch stepped on its own past the chunk reallocates its forcing buffers,
+    and a new z chunk changes the launches.  The ensemble recaptures its graphs
+    and every member still equals its eager solo run, bit for bit."""
+    say("ensemble after batch changes: 2 x 32^3, chunk 2")
+    p = params(32)
+    oms, seeds = [1.0, 4.0], [3, 4]
+    ics = [initial.initial_state(p, seed=40 + m) for m in range(2)]
+    A = lib.Batch(p, 2, oms)
+    for m in range(2):
+        A.set_member(m, ics[m])
+    assert A.pruned()
+    ens = lib.Ensemble([A], chunk=2)
+    ens.set_seeds(0, seeds)
+    # 1. member 1 gets energy outside the truncation ellipsoid: dense path
+    rng = np.random.default_rng(7)
+    noisy = [f.copy() for f in ics[1]]
+    for f in noisy:  # interior kx planes hold free complex values of a real field
+        f[:, :, 1:-1] += 1e-3 * (rng.standard_normal(f[:, :, 1:-1].shape) +
+                                 1j * rng.standard_normal(f[:, :, 1:-1].shape))
+    A.set_member(1, noisy)
+    assert not A.pruned()
+    start = [ics[0], noisy]
+    ens.step(4)
+    ens.sync()
+    for m in range(2):
+        with lib.Context(Params(**{**p.__dict__, "omega": oms[m]})) as g:
+            g.set_state(start[m])
+            assert g.pruned() == (m == 0)
+            rs = g.step(4, seeds[m])
+            ref = g.get_state()
+        assert ens.get_seeds(0)[m] == rs
+        got = A.get_member(m)
+        if m == 1:  # dense in both runs: the same kernels
+            for x, y in zip(got, ref):
+                assert np.array_equal(x, y)
+        else:  # dense in the batch, pruned solo: the same values up to rounding
+            num = sum(float(np.sum(np.abs(x - y) ** 2)) for x, y in zip(got, ref))
+            den = sum(float(np.sum(np.abs(y) ** 2)) for y in ref)
+            assert (num / den) ** 0.5 < 1e-14
+    say("  dense member after capture: equal")
+    # 2. the batch stepped on its own past the chunk, then a new z chunk
+    #    (member 0 back on the ellipsoid: the batch returns to the pruned path)
+    A.set_member(0, ics[0])
+    assert not A.pruned()
+    A.set_member(1, ics[1])
+    assert A.pruned()
+    s_after = A.step_batch(5, seeds)
+    A.set_zchunk(8)
+    ens.set_seeds(0, s_after)
+    ens.step(3)
+    ens.sync()
+    for m in range(2):
+        ref, rs = solo(Params(**{**p.__dict__, "omega": oms[m]}), ics[m], seeds[m], 8)
+        assert ens.get_seeds(0)[m] == rs
+        for x, y in zip(A.get_member(m), ref):
+            assert np.array_equal(x, y)
+    say("  batch steps past the chunk and a new z chunk: equal")
+    ens.close()
+    A.close()
