@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -75,6 +76,7 @@ struct gt_ctx {
   long long fs = 0;
   int prec = 0;
   int resident_blocks = 1024;
+  int dev = 0;  // the device current at creation, which holds the context's memory
   int zchunk = 0;   // z planes per chunk of passes B, C (and D) (0: all at once)
   bool zchunk_d = true;  // pass D runs in the chunks too (else once, after them)
   hipStream_t ext_stream = nullptr;  // an ensemble's stream the batch also runs on
@@ -688,8 +690,8 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
   chk(hipStreamCreate(&c->stream), "hipStreamCreate");
   {
     int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+    if (hipGetDevice(&dev) == hipSuccess) c->dev = dev;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev) == hipSuccess && ncu > 0)
       c->resident_blocks = 4 * ncu;
   }
   const size_t cw = c->prec ? sizeof(cx<float>) : sizeof(cx<double>);
@@ -978,6 +980,84 @@ int run_diag(gt_ctx* c, int member, const void* nl, std::vector<double>& host) {
   return GT_OK;
 }
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Diagnostics without allocation (gt_spectra_noalloc).
+//
+// The per-block partial sums of the diagnostics sweep and their total live in
+// `scratch`, a file-scope device array of fixed size instead of the buffers
+// gt_spectra allocates on its first call.  Every GPU the library runs on holds
+// its own copy of the array until the process ends.  Host code must not use
+// the array's name as a device address: scratch_ptr() takes the address from
+// hipGetSymbolAddress once per device (on the entry point's first call with
+// that device current) and every launch gets that pointer.  The kernels write
+// every partial and every total they later read, so nothing is cleared
+// between calls.  Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int NA_BLOCKS = 1024;           // partial-sum blocks (gt_spectra uses max(64, 4 CUs) = 1024 on MI355X)
+constexpr int NA_LEN = 4 * 512 + 18;      // diag_len of the largest binning (NX/2 = 512)
+constexpr size_t NA_DOUBLES = (size_t)NA_BLOCKS * NA_LEN + NA_LEN;  // partials, then the totals
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+void* na_ptr[NA_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+namespace {
+// The device address of `scratch` on device dev, which must be current.
+int scratch_ptr(int dev, double** out) {
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(GT_EINVAL, "spectra_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lk(na_mutex);
+  if (!na_ptr[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev)
+      return fail(GT_EINVAL, "spectra_noalloc: make the context's device " + std::to_string(dev) +
+                                 " current (device " + std::to_string(cur) + " is)");
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(GT_EHIP, "spectra_noalloc: no device address for the scratch array");
+    na_ptr[dev] = p;
+  }
+  *out = (double*)na_ptr[dev];
+  return GT_OK;
+}
+}  // namespace
+
+extern "C" int gt_spectra_noalloc(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->nmem != 1) return fail(GT_EINVAL, "gt_spectra_noalloc: single-member contexts only");
+  const int nbin = c->nx / 2;
+  const int len = diag_len(nbin);
+  if (nbin > 512 || len > NA_LEN)
+    return fail(GT_ESIZE, "gt_spectra_noalloc: NX/2 = " + std::to_string(nbin) + " above the 512 bins of the scratch");
+  // the same block count as gt_spectra, so that the sums are added in the same order
+  const int nblocks = std::max(64, c->resident_blocks);
+  if (nblocks > NA_BLOCKS)
+    return fail(GT_ESIZE, "gt_spectra_noalloc: " + std::to_string(nblocks) + " partial-sum blocks above the " +
+                              std::to_string(NA_BLOCKS) + " of the scratch");
+  double* base = nullptr;
+  if (int rc = scratch_ptr(c->dev, &base)) return rc;
+  double* partial = base;
+  double* total = base + (size_t)NA_BLOCKS * NA_LEN;
+  const KP kp = make_kp1(c, false, 0);
+  HIPCHK(launch_diag(kp, c->b, nullptr, partial, nblocks, total, nbin, c->stream));
+  double h[NA_LEN];
+  HIPCHK(hipMemcpyAsync(h, total, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nbin; q++) {
+    if (sp) sp[q] = h[q];
+    if (sp2d) sp2d[q] = h[nbin + q];
+    if (hel) hel[q] = h[2 * nbin + q];
+  }
+  if (glob)
+    for (int q = 0; q < 18; q++) glob[q] = h[4 * nbin + q];
+  if (gt_check_finite(h + 4 * nbin, 18) != GT_OK)
+    return fail(GT_ENONFINITE, "spectra_noalloc: the state is not finite (" + g_err + ")");
+  return GT_OK;
+}
 
 extern "C" int gt_check_finite(const double* v, int n) {
   if (!v && n > 0) return fail(GT_EINVAL, "gt_check_finite: null values");

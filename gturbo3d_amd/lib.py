@@ -28,7 +28,7 @@ EXPORTS = [
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
     "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
-    "gt_check_finite",
+    "gt_check_finite", "gt_spectra_noalloc",
 ]
 
 GT_ENONFINITE = 6
@@ -113,6 +113,7 @@ def load():
     L.gt_adjoint_forward.argtypes = [vp, ip, i32p, ip]
     L.gt_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ip]
     L.gt_check_finite.argtypes = [vp, ip]
+    L.gt_spectra_noalloc.argtypes = [vp, vp, vp, vp, vp]
     L.gt_last_error.restype = ctypes.c_char_p
     L.gt_last_error.argtypes = []
     _lib = L
@@ -244,6 +245,14 @@ class Context:
         nb = self.p.nx // 2
         sp, sp2d, hel, glob = np.zeros(nb), np.zeros(nb), np.zeros(nb), np.zeros(18)
         _check(load().gt_spectra(self.h, _ptr(sp), _ptr(sp2d), _ptr(hel), _ptr(glob)), "gt_spectra")
+        return sp, sp2d, hel, glob
+
+    def spectra_noalloc(self):
+        """spectra() with the partial sums in the library's fixed device array
+        (gt_spectra_noalloc): no device allocation."""
+        nb = self.p.nx // 2
+        sp, sp2d, hel, glob = np.zeros(nb), np.zeros(nb), np.zeros(nb), np.zeros(18)
+        _check(load().gt_spectra_noalloc(self.h, _ptr(sp), _ptr(sp2d), _ptr(hel), _ptr(glob)), "gt_spectra_noalloc")
         return sp, sp2d, hel, glob
 
     def fluxes(self):

@@ -106,6 +106,18 @@ int gt_init_velocity(gt_ctx* c, const double* ux, const double* uy, const double
  * divergence flag of a run, at the cadence the caller asks for diagnostics.) */
 int gt_spectra(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob);
 
+/* gt_spectra without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory, such as tight serving loops or
+ * code that must not call hipMalloc.  The per-block partial sums live in a
+ * fixed device array of the library (16.9 MB, held on every GPU the library
+ * runs on until the process ends) instead of buffers gt_spectra allocates on
+ * its first call; the outputs are those of gt_spectra, bit for bit on a GPU
+ * with at most 256 compute units.  Single-member contexts; NX/2 <= 512 bins
+ * and at most 1024 partial-sum blocks (4 per compute unit), else GT_ESIZE.
+ * The context's device must be current.  Calls that overlap on one GPU
+ * share the array and race: serialise them. */
+int gt_spectra_noalloc(gt_ctx* c, double* sp, double* sp2d, double* hel, double* glob);
+
 /* Host only (no device work): GT_OK when the n values are all finite, else
  * GT_ENONFINITE with gt_last_error() naming the first one that is not.  It is
  * the test gt_spectra applies to its glob sums. */
