@@ -36,7 +36,7 @@ __global__ __launch_bounds__(T* P) void k_zinv_adj(KP kp, const cx<double>* __re
                                                    cx<double>* __restrict__ out, const cx<double>* __restrict__ tw) {
   using F = Fft<double, N, P, 1, +1, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<double> lds[N * T];
+  GT_DYN_LDS(cx<double>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int j = blockIdx.y;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
   constexpr int SY = (P <= 64) ? SYNC_WAVE : SYNC_BLOCK;
   using FI = Fft<double, N, P, 1, +1, L, SY>;
   using FF = Fft<double, N, P, 1, -1, L, SY>;
-  __shared__ cx<double> lds[9 * N];
+  GT_DYN_LDS(cx<double>, lds);  // 9 N elements
   const int tid = threadIdx.x, g = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   const int r0 = (z * kp.ny + y0) * kp.nxp;
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(T* P) void k_zfwd_adj(KP kp, const cx<double>* __re
                                                    const cx<double>* __restrict__ lam) {
   using F = Fft<double, N, P, 1, -1, LayInterleaved<T>>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<double> lds[N * T];
+  GT_DYN_LDS(cx<double>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int j = blockIdx.y;
@@ -417,9 +417,9 @@ static hipError_t zinv_adj_n(const KP& kp, const void* nu, double s, bool ze, vo
   dim3 grid((kp.kxcut + T) / T, kp.ny, 3);
   const cx<double>* x = (const cx<double>*)nu;
   if (ze)
-    k_zinv_adj<N, T, P, true><<<grid, T * P, 0, st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
+    k_zinv_adj<N, T, P, true><<<grid, T * P, N * T * sizeof(cx<double>), st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
   else
-    k_zinv_adj<N, T, P, false><<<grid, T * P, 0, st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
+    k_zinv_adj<N, T, P, false><<<grid, T * P, N * T * sizeof(cx<double>), st>>>(kp, x, s, (cx<double>*)out, (const cx<double>*)tw);
   return hipGetLastError();
 }
 
@@ -431,9 +431,9 @@ static hipError_t zfwd_adj_n(const KP& kp, const void* work, void* out, const vo
   const cx<double>* w = (const cx<double>*)work;
   const cx<double>* t = (const cx<double>*)tw;
   if (lam)
-    k_zfwd_adj<N, T, P, true><<<grid, T * P, 0, st>>>(kp, w, (cx<double>*)out, t, (const cx<double>*)lam);
+    k_zfwd_adj<N, T, P, true><<<grid, T * P, N * T * sizeof(cx<double>), st>>>(kp, w, (cx<double>*)out, t, (const cx<double>*)lam);
   else
-    k_zfwd_adj<N, T, P, false><<<grid, T * P, 0, st>>>(kp, w, (cx<double>*)out, t, nullptr);
+    k_zfwd_adj<N, T, P, false><<<grid, T * P, N * T * sizeof(cx<double>), st>>>(kp, w, (cx<double>*)out, t, nullptr);
   return hipGetLastError();
 }
 
@@ -485,7 +485,7 @@ hipError_t launch_xadj(const KP& kp, void* work, const void* tw, hipStream_t st)
   switch (kp.nx) {
 #define GT_CASE(N)                                                                                    \
   case N:                                                                                             \
-    k_xadj<N, row_P(N)><<<grid, 9 * row_P(N), 0, st>>>(kp, (cx<double>*)work, (const cx<double>*)tw); \
+    k_xadj<N, row_P(N)><<<grid, 9 * row_P(N), 9 * N * sizeof(cx<double>), st>>>(kp, (cx<double>*)work, (const cx<double>*)tw); \
     return hipGetLastError();
     GT_FOR_EACH_NX_ADJ(GT_CASE)
 #undef GT_CASE

@@ -47,7 +47,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   constexpr int SY = (P <= 64) ? SYNC_WAVE : SYNC_BLOCK;
   using FI = Fft<W, N, P, 1, +1, L, SY>;
   using FF = Fft<W, N, P, 1, -1, L, SY>;
-  __shared__ cx<W> lds[6 * N];
+  GT_DYN_LDS(cx<W>, lds);  // 6 N elements
   const int tid = threadIdx.x, t = tid % P;
   // the group of a thread is uniform over its wavefront when P is a multiple
   // of the wavefront size: a scalar, so row starts are scalar too
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(3 * P) void k_xc2r(KP kp, const cx<double>* __restr
                                                 const cx<double>* __restrict__ tw) {
   using L = LaySwz<N>;
   using FI = Fft<double, N, P, 1, +1, L>;
-  __shared__ cx<double> lds[3 * N];
+  GT_DYN_LDS(cx<double>, lds);  // 3 N elements
   const int tid = threadIdx.x, sq = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   const long long r0 = ((long long)z * kp.ny + y0) * kp.nxp;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(3 * P) void k_xr2c(KP kp, const double* __restrict_
                                                 const cx<double>* __restrict__ tw) {
   using L = LaySwz<N>;
   using FF = Fft<double, N, P, 1, -1, L>;
-  __shared__ cx<double> lds[3 * N];
+  GT_DYN_LDS(cx<double>, lds);  // 3 N elements
   const int tid = threadIdx.x, sq = tid / P, t = tid % P;
   const int y0 = 2 * blockIdx.x, z = blockIdx.y;
   const long long plane = (long long)kp.nx * kp.ny * kp.nz;
@@ -262,9 +262,9 @@ static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t
   constexpr int P = row_P(N);
   dim3 grid(kp.ny / 2, kp.zn, kp.nmem);
   if (pz_cut(N, kp.kxcut))
-    k_xfused<W, N, P, true><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_xfused<W, N, P, true><<<grid, 6 * P, 6 * N * sizeof(cx<W>), s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   else
-    k_xfused<W, N, P, false><<<grid, 6 * P, 0, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_xfused<W, N, P, false><<<grid, 6 * P, 6 * N * sizeof(cx<W>), s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }
 
@@ -285,7 +285,7 @@ hipError_t launch_xc2r(const KP& kp, const void* work, double* phys, const void*
   switch (kp.nx) {
 #define GT_CASE(N)                                                                                             \
   case N:                                                                                                      \
-    k_xc2r<N, row_P(N)><<<grid, 3 * row_P(N), 0, s>>>(kp, (const cx<double>*)work, phys, (const cx<double>*)tw); \
+    k_xc2r<N, row_P(N)><<<grid, 3 * row_P(N), 3 * N * sizeof(cx<double>), s>>>(kp, (const cx<double>*)work, phys, (const cx<double>*)tw); \
     return hipGetLastError();
     GT_FOR_EACH_N(GT_CASE)
 #undef GT_CASE
@@ -299,7 +299,7 @@ hipError_t launch_xr2c(const KP& kp, const double* phys, void* work, const void*
   switch (kp.nx) {
 #define GT_CASE(N)                                                                                       \
   case N:                                                                                                \
-    k_xr2c<N, row_P(N)><<<grid, 3 * row_P(N), 0, s>>>(kp, phys, (cx<double>*)work, (const cx<double>*)tw); \
+    k_xr2c<N, row_P(N)><<<grid, 3 * row_P(N), 3 * N * sizeof(cx<double>), s>>>(kp, phys, (cx<double>*)work, (const cx<double>*)tw); \
     return hipGetLastError();
     GT_FOR_EACH_N(GT_CASE)
 #undef GT_CASE

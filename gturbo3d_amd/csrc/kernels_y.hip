@@ -17,7 +17,7 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, SIGN, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<W> lds[N * T];
+  GT_DYN_LDS(cx<W>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   const int kx = blockIdx.x * T + p;
   const int z = kp.z0 + blockIdx.y;
@@ -75,13 +75,13 @@ static hipError_t yfft_nt(int sign, int nfields, const KP& kp, void* work, const
   const cx<W>* t = (const cx<W>*)tw;
   if constexpr (pz_len(N)) {
     if (kp.pruned && pz_cut(N, kp.kycut)) {
-      if (sign > 0) k_yfft<W, N, T, P, +1, true><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
-      else k_yfft<W, N, T, P, -1, true><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
+      if (sign > 0) k_yfft<W, N, T, P, +1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
+      else k_yfft<W, N, T, P, -1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
       return hipGetLastError();
     }
   }
-  if (sign > 0) k_yfft<W, N, T, P, +1><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
-  else k_yfft<W, N, T, P, -1><<<grid, T * P, 0, s>>>(kp, w, t, nfields);
+  if (sign > 0) k_yfft<W, N, T, P, +1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
+  else k_yfft<W, N, T, P, -1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
   return hipGetLastError();
 }
 

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, +1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<W> lds[N * T];
+  GT_DYN_LDS(cx<W>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   // 1-D grid over (column, q).  Workgroups go to the 8 XCDs in turn, so the
   // nout blocks of one column are given the consecutive slots of one XCD
@@ -216,7 +216,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, -1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
-  __shared__ cx<W> lds[N * T];
+  GT_DYN_LDS(cx<W>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   // 1-D grid over columns (kx tile, ky, member).  When a tile row is shorter
   // than a 128-byte line (FP32 work fields, 8 pencils), PF adjacent tiles
@@ -358,13 +358,13 @@ static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, con
   const cx<W>* t = (const cx<W>*)tw;
   if constexpr (pz_len(N)) {
     if (kp.pruned && pz_cut(N, kp.kzcut)) {
-      if (pair) k_zinv<W, N, T, P, false, true, GT_ZINV_PAIR != 0><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
-      else k_zinv<W, N, T, P, false, true><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
+      if (pair) k_zinv<W, N, T, P, false, true, GT_ZINV_PAIR != 0><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
+      else k_zinv<W, N, T, P, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
       return hipGetLastError();
     }
   }
-  if (pair) k_zinv<W, N, T, P, false, false, GT_ZINV_PAIR != 0><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
-  else k_zinv<W, N, T, P><<<grid, T * P, 0, s>>>(kp, bb, w, t, nb);
+  if (pair) k_zinv<W, N, T, P, false, false, GT_ZINV_PAIR != 0><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
+  else k_zinv<W, N, T, P><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
   return hipGetLastError();
 }
 
@@ -390,26 +390,26 @@ static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b
     // the step epilogues of the pruned step: dropped output slots
     if (kp.pruned && pz_cut(N, kp.kzcut) && (epi == EPI_STEP1 || epi == EPI_STEP2)) {
       if (epi == EPI_STEP1) {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
       } else {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
       }
       return hipGetLastError();
     }
   }
   switch (epi) {
     case EPI_STEP1:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
       break;
     case EPI_STEP2:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, 0, s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
       break;
-    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
-    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, 0, s>>>(kp, w, bb, o, t); break;
+    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t); break;
+    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -445,7 +445,7 @@ hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* 
   case N: {                                                                                            \
     constexpr int T = pencil_T(N), P = pencil_P(N);                                                    \
     const int grid = (kp.kxcut + T) / T * kp.ny * 6;                                                   \
-    k_zinv<double, N, T, P, true><<<grid, T * P, 0, s>>>(kp, (const cx<double>*)b, (cx<double>*)work,  \
+    k_zinv<double, N, T, P, true><<<grid, T * P, N * T * sizeof(cx<double>), s>>>(kp, (const cx<double>*)b, (cx<double>*)work,  \
                                                           (const cx<double>*)tw, 6);                   \
     return hipGetLastError();                                                                          \
   }
