@@ -5,9 +5,15 @@ Workload: 512^3 triply periodic box, FP64, nu = 5e-5, Omega = 2, 3D3C
 forcing on the shell (2, 3] with injection rate 0.1, dt = 1e-3, seeded
 initial spectrum k^4 exp(-2 (k/4)^2) with E = 0.5.  A step is one full
 pass of the reference time-loop body (nlt, step1, nlt, step2, forcing) on
-the state resident in HBM.  Prints one JSON line.
+the state resident in HBM.  Prints one JSON line whose value is the FP64
+step; the same line carries the FP32-bulk step of the same workload
+("fp32_bulk": speed-up over FP64 and its own roofline) and the adjoint
+step of the 128^3 gradient workload ("adjoint": seconds per step, its
+fraction of HBM peak on the bytes its passes must move, a central-difference
+check).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32-bulk]
+                    [--legs fp64,fp32-bulk,adjoint]
 """
 import argparse
 import ctypes
@@ -204,7 +210,6 @@ def adjoint_bench(args):
         ad = sum(float(np.sum(np.real(np.conj(a) * c))) for a, c in zip(grad, d))
         worst = max(worst, abs(ad - fd) / abs(fd))
     t_fd = time.perf_counter() - t0
-    S = 16.0 * p.nz * p.ny * (p.nx // 2 + 1)
     bwd = float(np.median(t_bwd))
     print(json.dumps({
         "metric": "adjoint step s (128^3 FP64, 200 steps, checkpoints every 20)",
@@ -212,8 +217,6 @@ def adjoint_bench(args):
         "warmup": args.warmup, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
         "forward_s_per_run": float(np.median(t_fwd)), "backward_s_per_run": bwd,
         "gradient_s": float(np.median(t_fwd)) + bwd,
-        "model_bytes_per_adjoint_step": 270.0 * S,
-        "hbm_frac_model": 270.0 * S / (bwd / nsteps) / (HBM_PEAK_GBS * 1e9),
         "fd_check": {"directions": args.fd_directions, "eps": eps, "worst_rel_err": worst, "seconds": t_fd,
                      "directions_from": "initial_state seeds 100..115 scaled to |b0|"},
         "grad_amps": [float(v) for v in ga], "shells_k": [float(v) for v in g.forcing_shells()],
@@ -221,6 +224,94 @@ def adjoint_bench(args):
                    "objective": "kinetic energy in 8 <= |k| <= 12 at the final time",
                    "famp": p.famp, "tiles": dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), tiles))}}))
     g.close()
+
+
+def adjoint_leg(args, ngrad=2, fd_directions=4):
+    """The adjoint step of the 128^3 gradient workload (adjoint_bench's
+    configuration) for the main bench line: one warm-up gradient, then the
+    median backward time of ngrad gradients per step, a central-difference
+    check on fd_directions random directions, and the step's fraction of HBM
+    peak on the bytes its passes must move.
+
+    Those bytes come from the forward passes of the same context, counted
+    the way gt_prof_read counts them (every element a pass must read or
+    write, once; pruned extents).  With A, B, C, E one launch of the z-inverse,
+    y-inverse (6 fields), x and z-forward passes of one nonlinear evaluation:
+    the adjoint's z-inverse of Z emk b is A, the z-inverse of the cotangent
+    moves (A + E) / 3 (3 fields in at the kept modes, 3 out at the alive
+    columns), the y passes 9 and 6 fields (1.5 B + B), the x pass 9 in and
+    6 out (5/3 C), the z-forward 6 fields in and the cotangent in and out
+    (2 (A + E) / 3): 2A + E + 2.5 B + 5/3 C per step in reference mode.  The
+    recompute of each segment adds (every - 1) / every forward steps per
+    step, and keeping the segment's states one write and one read of three
+    fields at the kept modes."""
+    import numpy as np
+    import torch
+    from gturbo3d_amd import adjoint, initial, lib
+    p = adjoint_params()
+    nsteps, every, seed = 200, 20, 7
+    b0 = initial.initial_state(p, seed=0)
+    g = lib.Context(p)
+    g.set_state(b0)
+    tiles = g.autotune(3)
+    # forward pass bytes of this context (all passes whole: no z chunks)
+    g.set_zchunk(0)
+    g.prof_enable(True)
+    g.step(2, seed)
+    ms, launches, by = g.prof_read()
+    g.prof_enable(False)
+    g.set_state(b0)
+    per = {q: by[q] * launches[q] / 2.0 for q in range(lib.NPASS)}  # per forward step
+    A, B, C, E = by[0], by[1], by[2], by[4]
+    X = (2.0 * E - A) / 9.0  # 16 bytes x kept modes of one field
+    fwd_step = sum(per.values())
+    adj_bytes = 2 * A + E + 2.5 * B + (5.0 / 3.0) * C + (every - 1) / every * fwd_step + 2 * 3 * X
+    wt = torch.from_numpy(adjoint.band_weights(p, 8.0, 12.0))
+    nsh = len(g.forcing_shells())
+    t_fwd, t_bwd = [], []
+    for it in range(1 + ngrad):
+        x = adjoint.to_real(b0).requires_grad_(True)
+        amps = torch.full((nsh,), p.famp, dtype=torch.float64, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = adjoint.band_energy(adjoint.run(g, x, amps, nsteps, seed, every), wt)
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= 1:
+            t_fwd.append(t1 - t0)
+            t_bwd.append(t2 - t1)
+    grad = adjoint.to_complex(x.grad)
+    w = wt.numpy()[..., 0]
+
+    def J(b):
+        g.set_state(b)
+        g.step(nsteps, seed)
+        return sum(float(np.sum(w * np.abs(f) ** 2)) for f in g.get_state())
+
+    nb = math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in b0))
+    worst, eps = 0.0, 1e-6
+    for k in range(fd_directions):
+        d = initial.initial_state(p, seed=100 + k)
+        sc = nb / math.sqrt(sum(float(np.sum(np.abs(f) ** 2)) for f in d))
+        d = [sc * f for f in d]
+        fd = (J([a + eps * c for a, c in zip(b0, d)]) - J([a - eps * c for a, c in zip(b0, d)])) / (2 * eps)
+        ad = sum(float(np.sum(np.real(np.conj(a) * c))) for a, c in zip(grad, d))
+        worst = max(worst, abs(ad - fd) / abs(fd))
+    g.close()
+    bwd = float(np.median(t_bwd))
+    s_step = bwd / nsteps
+    return {
+        "metric": "adjoint step s (128^3 FP64, 200 steps, checkpoints every 20)",
+        "value": s_step, "unit": "s per adjoint step", "higher_is_better": False,
+        "backward_s_per_run": bwd, "forward_s_per_run": float(np.median(t_fwd)), "gradients_timed": ngrad,
+        "alg_bytes_per_step": adj_bytes, "alg_hbm_frac": adj_bytes / s_step / (HBM_PEAK_GBS * 1e9),
+        "fd_check": {"directions": fd_directions, "eps": eps, "worst_rel_err": worst},
+        "config": {"workload": "gradient of the kinetic energy in 8 <= |k| <= 12 after 200 steps at 128^3, "
+                               "nu=1e-3, Omega=2, 3D3C (2,3] rate 0.1 seed 7, dt=4e-3, reference stage 1",
+                   "tiles": dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), tiles))},
+    }
 
 
 def precision_bench(args):
@@ -282,6 +373,9 @@ def main():
     ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk[,with_d]]: force the pencil widths (and the z chunk of passes B-D) "
                          "instead of autotuning")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
+    ap.add_argument("--legs", default=None,
+                    help="step mode: comma-separated legs, the first giving the line's value (default "
+                         "fp64,fp32-bulk,adjoint; with --precision fp32-bulk only that precision)")
     ap.add_argument("--mode", choices=["step", "copy", "ensemble", "adjoint", "precision"], default="step",
                     help="copy: a plain 1 GiB device copy, the reference kernel for scaling rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE to bytes")
@@ -320,6 +414,32 @@ def main():
         precision_bench(args)
         return
 
+    if args.legs:
+        legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+    else:
+        legs = ["fp64", "fp32-bulk", "adjoint"] if args.precision == "fp64" else [args.precision]
+    line = step_leg(args, legs[0], with_cpu=args.cpu_baseline == "auto")
+    for leg in legs[1:]:
+        if leg in ("fp64", "fp32-bulk"):
+            sub = step_leg(args, leg, with_cpu=False)
+            key = leg.replace("-", "_")
+            line[key] = {k: sub[k] for k in ("value", "unit", "ms_per_step", "dtype", "alg_bytes_per_step",
+                                              "alg_hbm_frac", "pass_ms_per_step", "roofline", "state_finite")}
+            line[key]["config"] = {"tiles": sub["config"]["tiles"], "pruned": sub["config"]["pruned"]}
+            line[key]["speedup_vs_" + legs[0].replace("-", "_")] = line["ms_per_step"] / sub["ms_per_step"]
+        elif leg == "adjoint":
+            line["adjoint"] = adjoint_leg(args)
+    print(json.dumps(line))
+
+
+def step_leg(args, precision, with_cpu):
+    """One precision of the step workload: autotune, warm-up, the timed step
+    loop, a profiled repeat (per-pass times, the roofline of the dominant
+    pass) and, with with_cpu, the host baseline."""
+    import numpy as np
+    import torch
+    from gturbo3d_amd import initial, lib
+    args.precision = precision
     n = args.size
     p0 = workload_params(n, args.precision)
     # famp = sqrt(0.1 / nf): mean energy input inpe = nf * famp^2 = 0.1
@@ -413,7 +533,7 @@ def main():
         roofline["traffic"] = tb
 
     cpu = None
-    if args.cpu_baseline == "auto":
+    if with_cpu:
         threads = args.cpu_threads or min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1), 16)
         reps = 3 if n <= 512 else 1
         tc = cpu_baseline(p, b, seed0, threads, reps=reps, warmup=1)
@@ -434,7 +554,7 @@ def main():
         cpu["fft_gflops_1_thread"] = round(cpu_fft_gflops(nf, 1), 3)
         cpu["fft_gflops_sample"] = f"fft_dir at {nf}^3, 2.5 N log2 N nominal flop, median of 3"
 
-    line = {
+    return {
         "metric": METRIC,
         "value": gps,
         "unit": "grid-pt·steps/s",
@@ -465,8 +585,6 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
 
 
-if __name__ == "__main__":
-    main()
+if __name__ == "__main__":    main()

@@ -196,3 +196,54 @@ def test_ensemble_follows_batch_changes_after_capture(require_gpu):
     say("  batch steps past the chunk and a new z chunk: equal")
     ens.close()
     A.close()
+
+
+def test_ensemble_cfg4_shapes(require_gpu):
+    """The batched-ensemble workload at its shapes: 32 members at 128^3 and 16
+    at 96^3 (FP64, nu = 1e-3, dt = 4e-3, 3D3C (2,3] at rate 0.1, seeds 0..47,
+    Omega ~ U[0.5, 10] from PCG64(48)) in one ensemble of captured graphs.
+    After one replay and a remainder every member equals its eager solo run
+    bit for bit, and one member of each grid matches the oracle."""
+    import math as _m
+    rng = np.random.default_rng(48)
+    omegas = rng.uniform(0.5, 10.0, 48)
+    groups = [(128, 32, 0), (96, 16, 32)]
+    batches, ps, ics = [], [], []
+    say("ensemble 32 x 128^3 + 16 x 96^3, chunk 5, 6 steps")
+    for n, nm, off in groups:
+        p = params(n, famp=0.0)
+        p.famp = _m.sqrt(0.1 / lib.forcing_count(p)[0])
+        bt = lib.Batch(p, nm, omegas[off:off + nm])
+        ic = [initial.initial_state(p, seed=off + m) for m in range(nm)]
+        for m in range(nm):
+            bt.set_member(m, ic[m])
+        batches.append(bt)
+        ps.append(p)
+        ics.append(ic)
+    ens = lib.Ensemble(batches, chunk=5)
+    ens.set_seeds(0, np.arange(0, 32, dtype=np.int32))
+    ens.set_seeds(1, np.arange(32, 48, dtype=np.int32))
+    ens.step(6)
+    ens.sync()
+    for q, (n, nm, off) in enumerate(groups):
+        got_seeds = ens.get_seeds(q)
+        for m in range(nm):
+            pm = Params(**{**ps[q].__dict__, "omega": float(omegas[off + m])})
+            ref, rs = solo(pm, ics[q][m], off + m, 6)
+            assert got_seeds[m] == rs
+            got = batches[q].get_member(m)
+            for x, y in zip(got, ref):
+                assert np.array_equal(x, y), (n, m)
+        say(f"  {nm} members at {n}^3 equal their solo runs")
+        m = nm - 1
+        pm = Params(**{**ps[q].__dict__, "omega": float(omegas[off + m])})
+        ob, os_ = ol.Oracle(pm).step(ics[q][m], off + m, 6)
+        assert got_seeds[m] == os_
+        got = batches[q].get_member(m)
+        num = sum(float(np.sum(np.abs(x - y) ** 2)) for x, y in zip(got, ob))
+        den = sum(float(np.sum(np.abs(y) ** 2)) for y in ob)
+        say(f"  member {m} at {n}^3 against the oracle: {(num / den) ** 0.5:.2e}")
+        assert (num / den) ** 0.5 < 1e-12
+    ens.close()
+    for bt in batches:
+        bt.close()

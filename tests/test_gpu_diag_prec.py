@@ -103,3 +103,44 @@ def test_fp32_bulk_against_fp64_oracle(require_gpu, n, variant):
     assert max(e_fields) < 1e-5
     assert e_sp < 1e-5
     assert e_E < 1e-6
+
+
+def test_fp32_bulk_512_workload(require_gpu):
+    """The FP32-bulk step at the 512^3 shape of its workload (nu = 5e-5,
+    Omega = 2, 3D3C forcing (2,3] at rate 0.1, dt = 1e-3): one nonlinear
+    evaluation against the FP64 oracle, then 10 steps against the FP64 GPU
+    run from the same state and seed, per field and per spectrum."""
+    import math as _m
+    base = dict(nu=5e-5, omega=2.0, dt=1e-3, k1f=2.0, k2f=3.0, famp=0.0)
+    p64 = cfg(512, **base)
+    p64.famp = _m.sqrt(0.1 / lib.forcing_count(p64)[0])
+    p32 = cfg(512, **{**base, "famp": p64.famp, "precision": PREC_FP32_BULK})
+    say("fp32-bulk 512^3: nonlinear term and 10 steps against FP64")
+    b = initial.initial_state(p64, seed=0)
+    with lib.Context(p32) as g:
+        g.set_state(b)
+        assert g.pruned()
+        N32 = g.nonlinear()
+        s32 = g.step(10, 0)
+        b32 = g.get_state()
+        sp32, _, hel32, _ = g.spectra()
+    with lib.Context(p64) as g:
+        g.set_state(b)
+        s64 = g.step(10, 0)
+        b64 = g.get_state()
+        sp64, _, hel64, _ = g.spectra()
+    assert s32 == s64
+    No, _ = ol.Oracle(p64).nlt(b, b)
+    e_nl = rel_l2(N32, No)
+    del No
+    e_fields = [rel_l2([x], [y]) for x, y in zip(b32, b64)]
+    m = sp64 > 1e-12 * sp64.max()
+    e_sp = float(np.max(np.abs(sp32[m] - sp64[m]) / sp64[m]))
+    mh = np.abs(hel64) > 1e-6 * np.abs(hel64).max()
+    e_hel = float(np.max(np.abs(hel32[mh] - hel64[mh]) / np.abs(hel64[mh])))
+    say(f"  nlt vs oracle {e_nl:.2e}; fields after 10 steps {[f'{e:.2e}' for e in e_fields]}; "
+        f"energy spectrum {e_sp:.2e}; helicity spectrum {e_hel:.2e}")
+    assert e_nl < 1e-5
+    assert max(e_fields) < 1e-5
+    assert e_sp < 1e-5
+    assert e_hel < 1e-4
