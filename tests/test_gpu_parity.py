@@ -89,8 +89,8 @@ def test_one_step_large_matches_oracle(require_gpu, n):
 
 
 def test_nonlinear_512_matches_oracle(require_gpu):
-    """The full 512^3 shape against the oracle (the x pass runs persistent
-    blocks at this size)."""
+    """The full 512^3 shape against the oracle (the x pass runs three
+    wavefronts per row pair at this size)."""
     p = cfg(512, omega=2.0, nu=5e-5, dt=1e-3)
     say("nonlinear 512^3")
     b = initial.initial_state(p, seed=5)
