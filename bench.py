@@ -581,6 +581,8 @@ def step_leg(args, precision, with_cpu):
         "pass_ms_per_step": {lib.PASS_NAMES[q]: ms[q] / args.steps for q in range(lib.NPASS)},
         "pass_alg_gbs": {lib.PASS_NAMES[q]: (bytes_per[q] / (ms[q] / launches[q] * 1e-3) / 1e9 if launches[q] else 0.0)
                          for q in range(lib.NPASS)},
+        "pass_alg_bytes_per_launch": {lib.PASS_NAMES[q]: float(bytes_per[q]) for q in range(lib.NPASS)},
+        "pass_launches_per_step": {lib.PASS_NAMES[q]: float(launches[q]) / max(args.steps, 1) for q in range(lib.NPASS)},
         "state_finite": bool(finite),
         "roofline": roofline,
         "cpu_baseline": cpu,
