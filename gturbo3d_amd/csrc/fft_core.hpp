@@ -23,7 +23,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gt {
+
+// f(std::integral_constant<int, I>) for I = B .. E-1, unrolled at compile time
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for_fft(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for_fft<B + 1, E>(f);
+  }
+}
 
 template <typename T>
 struct alignas(2 * sizeof(T)) cx {
@@ -254,6 +265,37 @@ __device__ __forceinline__ void stage_sync() {
   }
 }
 
+// FP64 twiddles held in registers: the r = 1 table entry of butterfly
+// j = t + P*m of every stage s >= 1, at slot slot(s) + m.  A kernel that runs
+// several transforms of one length (inverse or forward: both read the same
+// entries) loads them once, with its data, so no stage waits for a table
+// load.
+template <typename T, int N, int P>
+struct TwRegs {
+  using PL = Plan<N>;
+  static constexpr int slot(int s) {
+    int v = 0;
+    for (int q = 1; q < s; q++) v += cdiv(N / PL::radix(q), P);
+    return v;
+  }
+  static constexpr int NW = slot(PL::S);
+  cx<T> w[NW > 0 ? NW : 1];
+  __device__ __forceinline__ void load(const cx<T>* __restrict__ tw, int t) {
+    static_assert(sizeof(T) == 8, "FP64 transforms form the powers of the r = 1 entry");
+    static_for_fft<1, PL::S>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      constexpr int Rs = PL::radix(s), Ns = PL::ns(s), NR = N / Rs, M = cdiv(NR, P);
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int j = t + P * m;
+        // a thread past the last butterfly reads entry 0 and never uses it
+        const int k = (NR % P == 0 || j < NR) ? j % Ns : 0;
+        w[slot(s) + m] = tw[PL::twoff(s) + k * (Rs - 1)];
+      }
+    });
+  }
+};
+
 // ---------------------------------------------------------------------------
 // The staged transform.
 // ---------------------------------------------------------------------------
@@ -276,9 +318,13 @@ struct Fft {
   __device__ static __forceinline__ int out_pos(int t, int m, int r) { return t + P * m + r * NRL; }
   __device__ static __forceinline__ bool out_ok(int t, int m) { return (NRL % P == 0) || (t + P * m < NRL); }
 
-  template <int s>
+  using Twr = TwRegs<T, N, P>;
+  static constexpr int twn(int s) { return Twr::slot(s); }
+  __device__ static __forceinline__ void load_tw(const cx<T>* __restrict__ tw, int t, Twr& R) { R.load(tw, t); }
+
+  template <int s, bool PRE = false>
   __device__ static __forceinline__ void stage(const L& lay, cx<T>* lds, const cx<T>* __restrict__ tw, int t,
-                                               cx<T> (&in)[G][E0], cx<T> (&out)[G][EL]) {
+                                               cx<T> (&in)[G][E0], cx<T> (&out)[G][EL], const Twr* twr = nullptr) {
     constexpr int R = PL::radix(s);
     constexpr int Ns = PL::ns(s);
     constexpr int NR = N / R;
@@ -325,7 +371,16 @@ struct Fft {
             // is formed, so two twiddles are live instead of R-1 (register
             // pressure sets the occupancy of the fused x pass); the products
             // add at most R-2 roundings to w^r
-            const cx<T> w1 = w[0];
+            cx<T> w1;
+            if constexpr (PRE) {
+              w1 = twr->w[twn(s) + m];
+              // an empty asm makes w1 a value of this point of the stage, so
+              // that the powers formed from it are not hoisted to the kernel
+              // start and held in registers through every earlier stage
+              asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+            } else {
+              w1 = w[0];
+            }
             cx<T> wk = w1;
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -383,7 +438,7 @@ struct Fft {
         }
       }
       stage_sync<SYNC>();
-      stage<s + 1>(lay, lds, tw, t, in, out);
+      stage<s + 1, PRE>(lay, lds, tw, t, in, out, twr);
     }
   }
 
@@ -392,6 +447,11 @@ struct Fft {
   __device__ static __forceinline__ void run(const L& lay, cx<T>* lds, const cx<T>* __restrict__ tw, int t,
                                              cx<T> (&in)[G][E0], cx<T> (&out)[G][EL]) {
     stage<0>(lay, lds, tw, t, in, out);
+  }
+  // The same with the twiddles of load_tw (FP64).
+  __device__ static __forceinline__ void run(const L& lay, cx<T>* lds, const cx<T>* __restrict__ tw, const Twr& twr,
+                                             int t, cx<T> (&in)[G][E0], cx<T> (&out)[G][EL]) {
+    stage<0, true>(lay, lds, tw, t, in, out, &twr);
   }
 };
 

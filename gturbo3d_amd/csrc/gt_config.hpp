@@ -46,9 +46,12 @@ __host__ __device__ constexpr int alt_T(int n) {
 // x passes: threads per row sequence (three sequences per block)
 __host__ __device__ constexpr int row_P(int n) { return pencil_P(n); }
 
-// lengths the library is compiled for
+// lengths the library is compiled for (a build may restrict them with
+// -D'GT_FOR_EACH_N(X)=X(512)' to time kernel variants at one size)
+#ifndef GT_FOR_EACH_N
 #define GT_FOR_EACH_N(X) \
   X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+#endif
 
 inline bool supported_n(int n) {
   switch (n) {

@@ -40,6 +40,10 @@ namespace gt {
 #ifndef GT_XFUSED_WAVES_PER_EU
 #define GT_XFUSED_WAVES_PER_EU 1
 #endif
+// FP64 twiddles of the three-wave kernel held in registers (loaded once)
+#ifndef GT_X_PRETW
+#define GT_X_PRETW 1
+#endif
 template <typename W, int N, int P, bool PZ>
 __global__ __launch_bounds__(6 * P) __attribute__((amdgpu_waves_per_eu(GT_XFUSED_WAVES_PER_EU)))
 void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
@@ -188,13 +192,22 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   };
   // inverse transforms of u_c and w_c (rows y0 and y0+1 packed)
   cx<W> uo[1][FI::EL], wo[1][FI::EL];
+  typename FI::Twr twr;  // the same entries serve the forward transform
   {
     cx<W> iu[1][FI::E0], iw[1][FI::E0];
     load(work + c * kp.fs, iu);
     load(work + (3 + c) * kp.fs, iw);
-    FI::run(lay, lds, tw, t, iu, uo);
-    stage_sync<SY>();  // row c is free for the second transform
-    FI::run(lay, lds, tw, t, iw, wo);
+    if constexpr (GT_X_PRETW) {
+      // the twiddles of all three transforms, loaded with the data
+      FI::load_tw(tw, t, twr);
+      FI::run(lay, lds, tw, twr, t, iu, uo);
+      stage_sync<SY>();  // row c is free for the second transform
+      FI::run(lay, lds, tw, twr, t, iw, wo);
+    } else {
+      FI::run(lay, lds, tw, t, iu, uo);
+      stage_sync<SY>();
+      FI::run(lay, lds, tw, t, iw, wo);
+    }
   }
   stage_sync<SY>();
   // round 1: u_c to row c; Q = -u_{c+1} w_c
@@ -238,7 +251,8 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   __syncthreads();  // every w row has been read before the rows are reused
   // forward transform of P_{c+2} in row c, then its two rows unpacked
   cx<W> fout[1][FF::EL];
-  FF::run(lay, lds, tw, t, pr, fout);
+  if constexpr (GT_X_PRETW) FF::run(lay, lds, tw, twr, t, pr, fout);
+  else FF::run(lay, lds, tw, t, pr, fout);
   stage_sync<SY>();
   static_for<0, FF::ML * FF::RL>([&](auto qc) {
     constexpr int q = decltype(qc)::value, m = q / FF::RL, r = q % FF::RL;

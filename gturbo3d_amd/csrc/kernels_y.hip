@@ -10,6 +10,13 @@
 
 namespace gt {
 
+// FP64 twiddles loaded with the data (1) or by each stage (0).  Measured at
+// 512^3 with 64-plane z chunks: 1 makes pass B 5% slower (the other passes'
+// preloads pay; this one does not), so it is off.
+#ifndef GT_Y_PRETW
+#define GT_Y_PRETW 0
+#endif
+
 // PZ: the slots that mid_slot() prunes (kycut < 3N/8) are zeros known at
 // compile time on input (SIGN > 0) or never stored on output (SIGN < 0).
 template <typename W, int N, int T, int P, int SIGN, bool PZ = false>
@@ -52,7 +59,14 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
     }
     in[0][qs] = v;
   });
-  F::run(lay, lds, tw, t, in, out);
+  if constexpr (sizeof(W) == 8 && GT_Y_PRETW) {
+    // FP64: the twiddles are loaded with the data, so no stage waits for them
+    typename F::Twr twr;
+    F::load_tw(tw, t, twr);
+    F::run(lay, lds, tw, twr, t, in, out);
+  } else {
+    F::run(lay, lds, tw, t, in, out);
+  }
   if (!on) return;
   static_for<0, ML * RL>([&](auto qc) {
     constexpr int qs = decltype(qc)::value, m = qs / RL, r = qs % RL;

@@ -30,6 +30,21 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 #ifndef GT_ZFWD_PAIR
 #define GT_ZFWD_PAIR 1
 #endif
+// FP64 pass A: the transforms' twiddles in registers, loaded with the data
+// (1), before it (2) or read from the table by each stage (0)
+#ifndef GT_ZINV_PRETW
+#define GT_ZINV_PRETW 1
+#endif
+// FP64 pass E: each field's loads issued before the previous field's
+// transform (1) or when its own transform starts (0)
+#ifndef GT_ZFWD_PREF
+#define GT_ZFWD_PREF 1
+#endif
+// output slots per load batch of the FP64 epilogue: all 8 (6 live at 512)
+// spill; 4 does not
+#ifndef GT_ZFWD_EPB64
+#define GT_ZFWD_EPB64 4
+#endif
 
 // Pass A: one output component q per block (0..2: u = curl b,
 // 3..5: w = k^2 b), so a block holds only the one or two fields of b it
@@ -99,10 +114,19 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
   // FP64 work fields: all loads first, unconditionally from a valid element
   // (element 0 of the column for an unused slot), so every load is in flight
   // before the first wait; the selects come after.  FP32 work fields keep
-  // predicated loads (the 16 double values would cost the occupancy).
+  // predicated loads (below).
   constexpr bool BATCH = sizeof(W) == 8;
   cx<double> la[M0 * R0], lb[M0 * R0];
+  // FP64: the kz wavenumbers of the thread's slots and the transforms'
+  // twiddles are loaded with the data, so that nothing after the first wait
+  // waits for memory again
+  // (measured at 512; other lengths keep the table loads of each slot and
+  // stage, whose registers would cost them resident blocks)
+  constexpr bool PRELD = BATCH && N == 512;
+  double fzv[M0 * R0];
+  typename F::Twr twr;
   if constexpr (BATCH) {
+    if constexpr (PRELD && GT_ZINV_PRETW == 2) F::load_tw(tw, t, twr);
     static_for<0, M0 * R0>([&](auto qc) {
       constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
       if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NR0>())) {
@@ -115,46 +139,85 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         lb[qs] = curl ? ld_row(B + c, o) : cx<double>{0.0, 0.0};
       }
     });
-  }
-  static_for<0, M0 * R0>([&](auto qc) {
-    constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
-    if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
-      in[0][qs] = {W(0), W(0)};
-    } else {
-      const int n = F::in_pos(t, m, r);
-      const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
-      const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
-      cx<double> va = {0.0, 0.0}, vb = {0.0, 0.0};
-      if constexpr (BATCH) {
-        va = use ? la[qs] : cx<double>{0.0, 0.0};
-        vb = (use && curl) ? lb[qs] : cx<double>{0.0, 0.0};
-      } else if (use) {
-        const long long c = (long long)(P * m + r * F::NR0) * zs;
-        va = ld_row(A + c, lane);
-        if (curl) vb = ld_row(B + c, lane);
-      }
-      if constexpr (EMK) {
-        const bool k0 = (kx == 0) && (j == 0) && (n == 0);
-        const double e = k0 ? 0.0 : emk_at(kp, kx < kp.nxh ? kx : kp.nxh - 1, j, n, fx * fx + fy * fy + fz * fz);
-        va = {e * va.x, e * va.y};
-        vb = {e * vb.x, e * vb.y};
-      }
-      double re, im;
-      if (curl) {
-        const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
-        const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
-        re = -ca * va.y + cb * vb.y;
-        im = ca * va.x - cb * vb.x;
-      } else {
-        const double k2 = fx * fx + fy * fy + fz * fz;
-        re = k2 * va.x;
-        im = k2 * va.y;
-      }
-      in[0][qs] = {(W)re, (W)im};
+    if constexpr (PRELD) {
+      static_for<0, M0 * R0>([&](auto qc) {
+        constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+        if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NR0>())) fzv[qs] = kp.fkz[F::in_ok(t, m) ? F::in_pos(t, m, r) : 0];
+      });
+      if constexpr (GT_ZINV_PRETW == 1) F::load_tw(tw, t, twr);
     }
-  });
+  }
+  // kz wavenumber of slot qs (in_ok)
+  auto fz_of = [&](auto qc) -> double {
+    constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+    if constexpr (PRELD) return fzv[qs];
+    else return kp.fkz[F::in_pos(t, m, r)];
+  };
+  auto fft = [&](cx<W>(&i)[1][M0 * R0], cx<W>(&o)[1][ML * RL], const LI& ly) {
+    if constexpr (PRELD && GT_ZINV_PRETW) F::run(ly, lds, tw, twr, t, i, o);
+    else F::run(ly, lds, tw, t, i, o);
+  };
+  // the transform input of slot qs from the loaded b values (zero where the
+  // slot is unused)
+  auto form = [&](auto qc, cx<double> va, cx<double> vb, double fz) {
+    constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+    const int n = F::in_pos(t, m, r);
+    if constexpr (EMK) {
+      const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+      const double e = k0 ? 0.0 : emk_at(kp, kx < kp.nxh ? kx : kp.nxh - 1, j, n, fx * fx + fy * fy + fz * fz);
+      va = {e * va.x, e * va.y};
+      vb = {e * vb.x, e * vb.y};
+    }
+    double re, im;
+    if (curl) {
+      const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
+      const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
+      re = -ca * va.y + cb * vb.y;
+      im = ca * va.x - cb * vb.x;
+    } else {
+      const double k2 = fx * fx + fy * fy + fz * fz;
+      re = k2 * va.x;
+      im = k2 * va.y;
+    }
+    in[0][qs] = {(W)re, (W)im};
+  };
+  if constexpr (BATCH) {
+    static_for<0, M0 * R0>([&](auto qc) {
+      constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+      if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
+        in[0][qs] = {W(0), W(0)};
+      } else {
+        const int n = F::in_pos(t, m, r);
+        const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
+        const double fz = F::in_ok(t, m) ? fz_of(qc) : 0.0;
+        form(qc, use ? la[qs] : cx<double>{0.0, 0.0}, (use && curl) ? lb[qs] : cx<double>{0.0, 0.0}, fz);
+      }
+    });
+  } else {
+    // FP32 work fields: predicated loads, slot by slot.  Batching them (all
+    // loads of 4 slots issued before the first wait), with the paired k^2
+    // input kept in registers instead of b read again, takes 81 registers
+    // and halves the resident blocks: 10% slower at 16 pencils.
+    static_for<0, M0 * R0>([&](auto qc) {
+      constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+      if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
+        in[0][qs] = {W(0), W(0)};
+      } else {
+        const int n = F::in_pos(t, m, r);
+        const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
+        const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
+        cx<double> va = {0.0, 0.0}, vb = {0.0, 0.0};
+        if (use) {
+          const long long c = (long long)(P * m + r * F::NR0) * zs;
+          va = ld_row(A + c, lane);
+          if (curl) vb = ld_row(B + c, lane);
+        }
+        form(qc, va, vb, fz);
+      }
+    });
+  }
   LI lay{p};
-  F::run(lay, lds, tw, t, in, out);
+  fft(in, out, lay);
   auto store = [&](int comp) {
     cx<W>* dst = work + comp * kp.fs + tile;
 #pragma unroll
@@ -171,21 +234,24 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
       constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
       if constexpr (PZ && mid_slot<N, P, P * m + r * F::NR0>()) {
         in[0][qs] = {W(0), W(0)};
+      } else if constexpr (BATCH) {
+        const int n = F::in_pos(t, m, r);
+        const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
+        const double fz = F::in_ok(t, m) ? fz_of(qc) : 0.0;
+        const cx<double> vb = use ? lb[qs] : cx<double>{0.0, 0.0};
+        const double k2 = fx * fx + fy * fy + fz * fz;
+        in[0][qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
       } else {
         const int n = F::in_pos(t, m, r);
         const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
         const double fz = F::in_ok(t, m) ? kp.fkz[n] : 0.0;
         cx<double> vb = {0.0, 0.0};
-        if constexpr (BATCH) {
-          vb = use ? lb[qs] : cx<double>{0.0, 0.0};
-        } else if (use) {
-          vb = ld_row(B + (long long)(P * m + r * F::NR0) * zs, lane);
-        }
+        if (use) vb = ld_row(B + (long long)(P * m + r * F::NR0) * zs, lane);
         const double k2 = fx * fx + fy * fy + fz * fz;
         in[0][qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
       }
     });
-    F::run(lay, lds, tw, t, in, out);
+    fft(in, out, lay);
     if (K < 0) return;
     store(3 + fb);
   } else {
@@ -205,6 +271,11 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
 #endif
 #ifndef GT_ZFWD_WPE32
 #define GT_ZFWD_WPE32 4
+#endif
+// output slots per load batch of the FP32 epilogue: 1 (2 spills and makes
+// the pass 25% slower)
+#ifndef GT_ZFWD_EPB32
+#define GT_ZFWD_EPB32 1
 #endif
 // PZ: the output slots that mid_slot() prunes (kzcut < 3N/8) are never
 // stored, so the transform's last stage drops them.
@@ -248,20 +319,55 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   const unsigned lane = p + t * zs;
   LI lay{p};
 
+  // FP64 (PRE): the transforms' twiddles are loaded once, with the first
+  // field, and each field's loads are issued before the previous field's
+  // transform, so they are in flight while it runs (the LDS barriers of a
+  // transform wait for LDS operations only).  FP32 loads each field when
+  // its transform starts (registers set its occupancy).
+  // (measured at 512; other lengths keep the loads of the original order,
+  // whose registers would cost them resident blocks or spills)
+  constexpr bool TUNED = sizeof(W) == 8 && N == 512;
+  constexpr bool PRE = TUNED && GT_ZFWD_PREF;
+  typename F::Twr twr;
+  auto load_field = [&](int f, cx<W>(&v)[M0 * R0]) {
+    const cx<W>* src = work + f * kp.fs + tile;
+    static_for<0, M0 * R0>([&](auto qc) {
+      constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
+      const cx<W>* row = src + (long long)(P * m + r * F::NR0) * zs;
+      if constexpr (PRE) {
+        // unconditional, from a valid element (plane P*m + r*N/R0 < N); the
+        // values of unused slots and of pencils past the x cut are dropped
+        v[qs] = ld_row_p<NT_E>(row, F::in_ok(t, m) ? lane : (unsigned)p);
+      } else {
+        cx<W> x = {W(0), W(0)};
+        if (K >= 0 && F::in_ok(t, m)) x = ld_row_p<NT_E>(row, lane);
+        v[qs] = x;
+      }
+    });
+  };
   cx<W> y[3][ML * RL];
+  cx<W> nx[M0 * R0];
+  if constexpr (PRE) {
+    load_field(0, nx);
+    F::load_tw(tw, t, twr);
+  }
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     cx<W> in[1][M0 * R0], o[1][ML * RL];
-    const cx<W>* src = work + f * kp.fs + tile;
+    if constexpr (PRE) {
 #pragma unroll
-    for (int m = 0; m < M0; m++)
+      for (int m = 0; m < M0; m++)
 #pragma unroll
-      for (int r = 0; r < R0; r++) {
-        cx<W> v = {W(0), W(0)};
-        if (K >= 0 && F::in_ok(t, m)) v = ld_row_p<NT_E>(src + (long long)(P * m + r * F::NR0) * zs, lane);
-        in[0][m * R0 + r] = v;
-      }
-    F::run(lay, lds, tw, t, in, o);
+        for (int r = 0; r < R0; r++)
+          in[0][m * R0 + r] = (K >= 0 && F::in_ok(t, m)) ? nx[m * R0 + r] : cx<W>{W(0), W(0)};
+      if (f < 2) load_field(f + 1, nx);
+      F::run(lay, lds, tw, twr, t, in, o);
+    } else {
+      load_field(f, nx);
+#pragma unroll
+      for (int q = 0; q < M0 * R0; q++) in[0][q] = nx[q];
+      F::run(lay, lds, tw, t, in, o);
+    }
 #pragma unroll
     for (int q = 0; q < ML * RL; q++) y[f][q] = o[0][q];
   }
@@ -272,74 +378,113 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   const double fxs = fx * fx, fys = fy * fy;
   const double kxn = kp.fkxn[kxc], kyn = kp.fkyn[j];
   const double exy = VS ? kp.emx[kxc] * kp.emy[j] : 0.0;
-  static_for<0, ML * RL>([&](auto qc) {
-    constexpr int qs = decltype(qc)::value, m = qs / RL, r = qs % RL;
-    if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NRL>())) {
-      if (!F::out_ok(t, m)) return;
-      const int n = F::out_pos(t, m, r);
-      if (!keep_idx(n, N, K)) return;
-      const long long c = tile + (long long)(P * m + r * F::NRL) * zs;  // + lane
-      const double fz = kp.fkz[n];
-      const bool k0 = (kx == 0) && (j == 0) && (n == 0);
-      cx<double> a[3], bv[3];
+  // The epilogue in batches of EPB output slots: every load of a batch (b
+  // and the kz tables) is issued before the first is waited for.  A slot
+  // that is not stored reads the column's kz = 0 element instead of its own
+  // (a line the kept slots fetch anyway).
+  constexpr int NS = ML * RL;
+  constexpr int EPB0 = TUNED ? GT_ZFWD_EPB64 : sizeof(W) == 8 ? 1 : GT_ZFWD_EPB32;
+  constexpr int EPB = (NS % EPB0 != 0) ? NS : EPB0;
+  static_for<0, NS / EPB>([&](auto bc) {
+    constexpr int q0 = decltype(bc)::value * EPB;
+    cx<double> bvs[EPB][3];
+    double fzs[EPB], fzns[EPB], ezs[EPB];
+    static_for<0, EPB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value, qs = q0 + i, m = qs / RL, r = qs % RL;
+      if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NRL>())) {
+        const int n = F::out_ok(t, m) ? F::out_pos(t, m, r) : 0;
+        const bool ok = F::out_ok(t, m) && keep_idx(n, N, K);
+        if constexpr (TUNED) {
+          if constexpr (EPI != EPI_INIT) {
+            const long long c = ok ? tile + (long long)(P * m + r * F::NRL) * zs : tile;
+            const unsigned o = ok ? lane : (unsigned)p;
 #pragma unroll
-      for (int f = 0; f < 3; f++) a[f] = {(double)y[f][qs].x * kp.scale, (double)y[f][qs].y * kp.scale};
-      const double k2 = fxs + fys + fz * fz;
-      if constexpr (EPI == EPI_INIT) {
-        // incompr (src/trunc.f90:47-63), k = 0 skipped
+            for (int f = 0; f < 3; f++) bvs[i][f] = ld_row(b + f * kp.fs + c, o);
+          }
+          fzs[i] = kp.fkz[n];
+          fzns[i] = kp.fkzn[n];
+          if constexpr (VS) ezs[i] = kp.emz[n];
+        } else if (ok) {
+          // only the stored slots load, and the trunc and integrating-factor
+          // tables are read where they are used (registers set the occupancy)
+          if constexpr (EPI != EPI_INIT) {
+            const long long c = tile + (long long)(P * m + r * F::NRL) * zs;
+#pragma unroll
+            for (int f = 0; f < 3; f++) bvs[i][f] = ld_row(b + f * kp.fs + c, lane);
+          }
+          fzs[i] = kp.fkz[n];
+        }
+      }
+    });
+    static_for<0, EPB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value, qs = q0 + i, m = qs / RL, r = qs % RL;
+      if constexpr (!(PZ && mid_slot<N, P, P * m + r * F::NRL>())) {
+        if (!F::out_ok(t, m)) return;
+        const int n = F::out_pos(t, m, r);
+        if (!keep_idx(n, N, K)) return;
+        const long long c = tile + (long long)(P * m + r * F::NRL) * zs;  // + lane
+        const double fz = fzs[i];
+        const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+        cx<double> a[3], bv[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) a[f] = {(double)y[f][qs].x * kp.scale, (double)y[f][qs].y * kp.scale};
+        const double k2 = fxs + fys + fz * fz;
+        if constexpr (EPI == EPI_INIT) {
+          // incompr (src/trunc.f90:47-63), k = 0 skipped
+          if (!k0) {
+            double sr = fx * a[0].x + fy * a[1].x;
+            double si = fx * a[0].y + fy * a[1].y;
+            sr = sr + fz * a[2].x;
+            si = si + fz * a[2].y;
+            a[0] = {a[0].x - sr * fx / k2, a[0].y - si * fx / k2};
+            a[1] = {a[1].x - sr * fy / k2, a[1].y - si * fy / k2};
+            a[2] = {a[2].x - sr * fz / k2, a[2].y - si * fz / k2};
+          }
+        } else {
+          // Coriolis with the start-of-step b (src/nlt.f90:67-81); b is read
+          // once and kept for the update below
+          const double of = omega * fz;
+#pragma unroll
+          for (int f = 0; f < 3; f++) {
+            bv[f] = bvs[i][f];
+            a[f] = {a[f].x - of * bv[f].y, a[f].y + of * bv[f].x};
+          }
+        }
+        // invcurl, k = 0 left untouched (src/produ.f90:83-99), with one
+        // reciprocal of k2 instead of three divisions
+        cx<double> nl[3] = {a[0], a[1], a[2]};
         if (!k0) {
-          double sr = fx * a[0].x + fy * a[1].x;
-          double si = fx * a[0].y + fy * a[1].y;
-          sr = sr + fz * a[2].x;
-          si = si + fz * a[2].y;
-          a[0] = {a[0].x - sr * fx / k2, a[0].y - si * fx / k2};
-          a[1] = {a[1].x - sr * fy / k2, a[1].y - si * fy / k2};
-          a[2] = {a[2].x - sr * fz / k2, a[2].y - si * fz / k2};
+          const double ik2 = 1.0 / k2;
+          const double qx = fx * ik2, qy = fy * ik2, qz = fz * ik2;
+          nl[0] = {-qy * a[2].y + qz * a[1].y, qy * a[2].x - qz * a[1].x};
+          nl[1] = {-qz * a[0].y + qx * a[2].y, qz * a[0].x - qx * a[2].x};
+          nl[2] = {-qx * a[1].y + qy * a[0].y, qx * a[1].x - qy * a[0].x};
         }
-      } else {
-        // Coriolis with the start-of-step b (src/nlt.f90:67-81); b is read
-        // once and kept for the update below
-        const double of = omega * fz;
+        if (kp.truncate == 1) {
+          // trunc (src/trunc.f90:14-19,27)
+          const double kn = ((TUNED ? fzns[i] : kp.fkzn[n]) + kyn) + kxn;
+          if (kn > kp.rtrunc2 || k0) {
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-          bv[f] = ld_row(b + f * kp.fs + c, lane);
-          a[f] = {a[f].x - of * bv[f].y, a[f].y + of * bv[f].x};
+            for (int f = 0; f < 3; f++) nl[f] = {0.0, 0.0};
+          }
         }
-      }
-      // invcurl, k = 0 left untouched (src/produ.f90:83-99), with one
-      // reciprocal of k2 instead of three divisions
-      cx<double> nl[3] = {a[0], a[1], a[2]};
-      if (!k0) {
-        const double ik2 = 1.0 / k2;
-        const double qx = fx * ik2, qy = fy * ik2, qz = fz * ik2;
-        nl[0] = {-qy * a[2].y + qz * a[1].y, qy * a[2].x - qz * a[1].x};
-        nl[1] = {-qz * a[0].y + qx * a[2].y, qz * a[0].x - qx * a[2].x};
-        nl[2] = {-qx * a[1].y + qy * a[0].y, qx * a[1].x - qy * a[0].x};
-      }
-      if (kp.truncate == 1) {
-        // trunc (src/trunc.f90:14-19,27)
-        const double kn = (kp.fkzn[n] + kyn) + kxn;
-        if (kn > kp.rtrunc2 || k0) {
+        if constexpr (EPI == EPI_NONLIN || EPI == EPI_INIT) {
 #pragma unroll
-          for (int f = 0; f < 3; f++) nl[f] = {0.0, 0.0};
+          for (int f = 0; f < 3; f++) st_row_p<NT_E>(out + f * kp.fs + c, lane, nl[f]);
+        } else {
+          const double e = VS ? exy * (TUNED ? ezs[i] : kp.emz[n]) : emk_of(kp, k2);
+#pragma unroll
+          for (int f = 0; f < 3; f++) {
+            const cx<double> bb = bv[f];
+            cx<double> o;
+            if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
+            else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
+            if (k0) o = {0.0, 0.0};
+            st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
+          }
         }
       }
-      if constexpr (EPI == EPI_NONLIN || EPI == EPI_INIT) {
-#pragma unroll
-        for (int f = 0; f < 3; f++) st_row_p<NT_E>(out + f * kp.fs + c, lane, nl[f]);
-      } else {
-        const double e = VS ? exy * kp.emz[n] : emk_of(kp, k2);
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-          const cx<double> bb = bv[f];
-          cx<double> o;
-          if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
-          else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
-          if (k0) o = {0.0, 0.0};
-          st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
-        }
-      }
-    }
+    });
   });
 }
 
