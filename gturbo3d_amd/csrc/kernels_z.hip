@@ -35,6 +35,11 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 #ifndef GT_ZINV_PRETW
 #define GT_ZINV_PRETW 1
 #endif
+// FP32 pass A with PAIR: the second transform's input kept in registers (1)
+// or b read again for it (0)
+#ifndef GT_ZINV_IN2
+#define GT_ZINV_IN2 1
+#endif
 // FP64 pass E: each field's loads issued before the previous field's
 // transform (1) or when its own transform starts (0)
 #ifndef GT_ZFWD_PREF
@@ -158,7 +163,11 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
     else F::run(ly, lds, tw, t, i, o);
   };
   // the transform input of slot qs from the loaded b values (zero where the
-  // slot is unused)
+  // slot is unused); FP32 work fields with PAIR also keep the second
+  // transform's input, k^2 b of field fb, rounded to FP32 as it is formed,
+  // so b is not read again (GT_ZINV_IN2)
+  constexpr bool IN2 = PAIR && !BATCH && GT_ZINV_IN2;
+  cx<W> in2[M0 * R0];
   auto form = [&](auto qc, cx<double> va, cx<double> vb, double fz) {
     constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
     const int n = F::in_pos(t, m, r);
@@ -180,6 +189,13 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
       im = k2 * va.y;
     }
     in[0][qs] = {(W)re, (W)im};
+    if constexpr (IN2) {
+      const double k2 = fx * fx + fy * fy + fz * fz;
+      in2[qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
+      // held as FP32 from here (the conversion is not sunk to its use, where
+      // the double products would have to stay live through the transform)
+      asm volatile("" : "+v"(in2[qs].x), "+v"(in2[qs].y));
+    }
   };
   if constexpr (BATCH) {
     static_for<0, M0 * R0>([&](auto qc) {
@@ -241,6 +257,8 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         const cx<double> vb = use ? lb[qs] : cx<double>{0.0, 0.0};
         const double k2 = fx * fx + fy * fy + fz * fz;
         in[0][qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
+      } else if constexpr (IN2) {
+        in[0][qs] = in2[qs];
       } else {
         const int n = F::in_pos(t, m, r);
         const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
