@@ -296,6 +296,16 @@ struct TwRegs {
   }
 };
 
+// Copy the twiddle table of length N (Plan<N>::twsize() entries) into LDS,
+// cooperatively over nthr threads; a transform then reads its twiddles with
+// LDS latency instead of L2 latency (run() with the LDS copy as its table).
+// The copy is visible to the other wavefronts after the next block barrier.
+template <typename T, int N>
+__device__ __forceinline__ void tw_to_lds(const cx<T>* __restrict__ tw, cx<T>* twl, int tid, int nthr) {
+  constexpr int TS = Plan<N>::twsize();
+  for (int i = tid; i < TS; i += nthr) twl[i] = tw[i];
+}
+
 // ---------------------------------------------------------------------------
 // The staged transform.
 // ---------------------------------------------------------------------------

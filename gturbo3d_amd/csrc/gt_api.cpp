@@ -10,6 +10,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -1296,7 +1297,7 @@ int gt_autotune(gt_ctx* c, int nrep) {
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<gt_ctx::Rec> saved;
   saved.swap(c->recs);
-  std::vector<double> tA(NC, 1e30), tE(NC, 1e30), tY(NC, 1e30);
+  std::vector<double> tA(NC, 1e30), tE(NC, 1e30);
   int rc = GT_OK;
   auto ok = [](int n, int t) { return t == pencil_T(n) || (alt_T(n) > 0 && t == alt_T(n)); };
   auto enc = [](int n, int t) { return t == pencil_T(n) ? 0 : t; };
@@ -1327,7 +1328,6 @@ int gt_autotune(gt_ctx* c, int nrep) {
     c->recs.clear();
     if (ok(c->nz, w)) tA[ci] = std::min(tA[ci], ms[PASS_A] / nrep);
     if (ok(c->nz, w)) tE[ci] = std::min(tE[ci], ms[PASS_E1] / nrep);
-    if (ok(c->ny, w)) tY[ci] = std::min(tY[ci], (ms[PASS_B] + ms[PASS_D]) / nrep);
   }
   c->recs.swap(saved);
   c->prof = prof;
@@ -1339,24 +1339,33 @@ int gt_autotune(gt_ctx* c, int nrep) {
   };
   c->tz = pick(tA, c->nz);
   c->tze = pick(tE, c->nz);
-  c->ty = pick(tY, c->ny);
-  // z chunk of passes B-D with the chosen widths: wall time of whole
-  // evaluations (the chunks add launches, so per-pass sums would flatter them)
+  // width of the y passes and z chunk of passes B-D together, by the wall
+  // time of whole evaluations: the y passes share the memory-side cache with
+  // the x pass chunk by chunk, so their own event times (taken unchunked)
+  // picked widths that were slower in the chunked step, and the chunks add
+  // launches that per-pass sums would not count
   if (rc == GT_OK) {
-    // (planes, pass D in the chunks)
-    std::vector<std::pair<int, bool>> zcand = {{0, true}};
-    for (int d : {16, 8, 4, 2})
-      if (c->nz % d == 0 && c->nz / d >= 8)
-        for (bool wd : {true, false}) zcand.push_back({c->nz / d, wd});
+    // (y width, planes, pass D in the chunks)
+    std::vector<int> ycand;
+    for (int w : cand)
+      if (ok(c->ny, w)) ycand.push_back(enc(c->ny, w));
+    std::vector<std::tuple<int, int, bool>> zcand;
+    for (int yw : ycand) {
+      zcand.push_back({yw, 0, true});
+      for (int d : {16, 8, 4, 2})
+        if (c->nz % d == 0 && c->nz / d >= 8)
+          for (bool wd : {true, false}) zcand.push_back({yw, c->nz / d, wd});
+    }
     hipEvent_t e0 = pool_get(c), e1 = pool_get(c);
     if (!e0 || !e1) rc = fail(GT_EHIP, "autotune events");
-    const KP kp = make_kp(c, false);
     // two interleaved rounds over the candidates, the best time of each kept
     std::vector<double> tz(zcand.size(), 1e30);
     for (int round = 0; round < 2; round++)
       for (size_t i = 0; i < zcand.size() && rc == GT_OK; i++) {
-        c->zchunk = zcand[i].first;
-        c->zchunk_d = zcand[i].second;
+        c->ty = std::get<0>(zcand[i]);
+        c->zchunk = std::get<1>(zcand[i]);
+        c->zchunk_d = std::get<2>(zcand[i]);
+        const KP kp = make_kp(c, false);
         rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
         if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune record");
         for (int r = 0; r < nrep && rc == GT_OK; r++) rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);
@@ -1369,8 +1378,9 @@ int gt_autotune(gt_ctx* c, int nrep) {
     size_t bi = 0;
     for (size_t i = 1; i < zcand.size(); i++)
       if (tz[i] < tz[bi]) bi = i;
-    c->zchunk = zcand[bi].first;
-    c->zchunk_d = zcand[bi].second;
+    c->ty = std::get<0>(zcand[bi]);
+    c->zchunk = std::get<1>(zcand[bi]);
+    c->zchunk_d = std::get<2>(zcand[bi]);
     if (rc == GT_OK) c->tune_ms = tz[bi];  // the tuned time per evaluation
     if (e0) c->pool.push_back(e0);
     if (e1) c->pool.push_back(e1);

@@ -16,6 +16,12 @@ namespace gt {
   extern __shared__ __align__(16) unsigned char name##_raw[]; \
   T* name = reinterpret_cast<T*>(name##_raw)
 
+// FP32 transforms read their twiddles (R-1 per butterfly and stage) from a
+// copy of the table in LDS rather than from the global table (tw_to_lds)
+#ifndef GT_TWLDS32
+#define GT_TWLDS32 1
+#endif
+
 // f(std::integral_constant<int, I>) for I = B .. E-1, unrolled at compile time
 template <int B, int E, class F>
 __device__ __forceinline__ void static_for(F&& f) {

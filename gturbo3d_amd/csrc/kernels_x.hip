@@ -44,6 +44,11 @@ namespace gt {
 #ifndef GT_X_PRETW
 #define GT_X_PRETW 1
 #endif
+// FP32 six-wave kernel: twiddles from an LDS copy of the table (0: from the
+// global table, measured 2.5% faster at 512^3)
+#ifndef GT_X_TWLDS32
+#define GT_X_TWLDS32 0
+#endif
 template <typename W, int N, int P, bool PZ>
 __global__ __launch_bounds__(6 * P) __attribute__((amdgpu_waves_per_eu(GT_XFUSED_WAVES_PER_EU)))
 void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
@@ -69,6 +74,11 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   // three reads from LDS per position instead of four.
   constexpr bool REGU = GT_XFUSED_REGU && FI::RL == FF::R0 && FI::ML == FF::M0;
   cx<W> uo[1][FI::EL];
+  // FP32: the twiddles from a copy of the table in LDS behind the six rows
+  // (tw_to_lds), made visible by a block barrier before the first transform
+  // (GT_X_TWLDS32; measured 2.5% slower here, so off)
+  constexpr bool TWL = sizeof(W) == 4 && GT_TWLDS32 && GT_X_TWLDS32;
+  const cx<W>* twt = TWL ? lds + 6 * N : tw;
   // inverse transform of field w (u_x, u_y, u_z, w_x, w_y, w_z), rows y0 and y0+1
   {
     cx<W> in[1][FI::E0], out[1][FI::EL];
@@ -82,7 +92,11 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
       }
       in[0][q] = v;
     });
-    FI::run(lay, lds, tw, t, in, out);
+    if constexpr (TWL) {
+      tw_to_lds<W, N>(tw, lds + 6 * N, tid, 6 * P);
+      __syncthreads();
+    }
+    FI::run(lay, lds, twt, t, in, out);
     stage_sync<SY>();
 #pragma unroll
     for (int m = 0; m < FI::ML; m++)
@@ -121,7 +135,7 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
     }
   __syncthreads();  // every product read is done before any forward transform writes LDS
   if (SY == SYNC_BLOCK || w < 3) {
-    FF::run(lay, lds, tw, t, fin, fout);
+    FF::run(lay, lds, twt, t, fin, fout);
     stage_sync<SY>();
     static_for<0, FF::ML * FF::RL>([&](auto qc) {
       constexpr int q = decltype(qc)::value, m = q / FF::RL, r = q % FF::RL;
@@ -405,10 +419,11 @@ static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t
       k_xfused3<W, N, P, false><<<grid, 3 * P, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
     return hipGetLastError();
   }
+  const size_t lds6 = 6 * N * sizeof(cx<W>) + ((sizeof(W) == 4 && GT_TWLDS32 && GT_X_TWLDS32) ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
   if (pz_cut(N, kp.kxcut))
-    k_xfused<W, N, P, true><<<grid, 6 * P, 6 * N * sizeof(cx<W>), s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_xfused<W, N, P, true><<<grid, 6 * P, lds6, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   else
-    k_xfused<W, N, P, false><<<grid, 6 * P, 6 * N * sizeof(cx<W>), s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+    k_xfused<W, N, P, false><<<grid, 6 * P, lds6, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
   return hipGetLastError();
 }
 

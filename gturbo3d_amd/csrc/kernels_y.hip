@@ -16,6 +16,17 @@ namespace gt {
 #ifndef GT_Y_PRETW
 #define GT_Y_PRETW 0
 #endif
+// FP32 work fields: the twiddle table copied into LDS behind the transform's
+// rows at the block start, so every stage reads its R-1 twiddles with LDS
+// latency instead of L2 latency.  Measured at 512^3: passes B and D -11%
+// in FP32-bulk; with FP64 (one twiddle per stage) pass B +2%, so FP64 reads
+// the global table.
+template <typename W>
+constexpr bool y_twlds() { return sizeof(W) == 4 && GT_TWLDS32; }
+template <typename W, int N, int T>
+constexpr size_t yfft_lds() {
+  return (size_t)N * T * sizeof(cx<W>) + (y_twlds<W>() ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
+}
 
 // PZ: the slots that mid_slot() prunes (kycut < 3N/8) are zeros known at
 // compile time on input (SIGN > 0) or never stored on output (SIGN < 0).
@@ -48,6 +59,10 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
   LI lay{p};
 
   cx<W> in[1][M0 * R0], out[1][ML * RL];
+  // the stages after the first read their twiddles from this copy; the
+  // first stage's barrier makes it visible
+  cx<W>* twl = lds + N * T;
+  if constexpr (y_twlds<W>()) tw_to_lds<W, N>(tw, twl, tid, T * P);
   static_for<0, M0 * R0>([&](auto qc) {
     constexpr int qs = decltype(qc)::value, m = qs / R0, r = qs % R0;
     cx<W> v = {W(0), W(0)};
@@ -64,6 +79,8 @@ __global__ __launch_bounds__(T* P) void k_yfft(KP kp, cx<W>* work, const cx<W>* 
     typename F::Twr twr;
     F::load_tw(tw, t, twr);
     F::run(lay, lds, tw, twr, t, in, out);
+  } else if constexpr (y_twlds<W>()) {
+    F::run(lay, lds, twl, t, in, out);
   } else {
     F::run(lay, lds, tw, t, in, out);
   }
@@ -89,13 +106,13 @@ static hipError_t yfft_nt(int sign, int nfields, const KP& kp, void* work, const
   const cx<W>* t = (const cx<W>*)tw;
   if constexpr (pz_len(N)) {
     if (kp.pruned && pz_cut(N, kp.kycut)) {
-      if (sign > 0) k_yfft<W, N, T, P, +1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
-      else k_yfft<W, N, T, P, -1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
+      if (sign > 0) k_yfft<W, N, T, P, +1, true><<<grid, T * P, yfft_lds<W, N, T>(), s>>>(kp, w, t, nfields);
+      else k_yfft<W, N, T, P, -1, true><<<grid, T * P, yfft_lds<W, N, T>(), s>>>(kp, w, t, nfields);
       return hipGetLastError();
     }
   }
-  if (sign > 0) k_yfft<W, N, T, P, +1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
-  else k_yfft<W, N, T, P, -1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, t, nfields);
+  if (sign > 0) k_yfft<W, N, T, P, +1><<<grid, T * P, yfft_lds<W, N, T>(), s>>>(kp, w, t, nfields);
+  else k_yfft<W, N, T, P, -1><<<grid, T * P, yfft_lds<W, N, T>(), s>>>(kp, w, t, nfields);
   return hipGetLastError();
 }
 

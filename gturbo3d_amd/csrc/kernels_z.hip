@@ -158,8 +158,12 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
     if constexpr (PRELD) return fzv[qs];
     else return kp.fkz[F::in_pos(t, m, r)];
   };
+  // FP32: the twiddles from a copy of the table in LDS behind the pencils
+  // (tw_to_lds; the first stage's barrier makes it visible)
+  constexpr bool TWL = sizeof(W) == 4 && GT_TWLDS32;
   auto fft = [&](cx<W>(&i)[1][M0 * R0], cx<W>(&o)[1][ML * RL], const LI& ly) {
     if constexpr (PRELD && GT_ZINV_PRETW) F::run(ly, lds, tw, twr, t, i, o);
+    else if constexpr (TWL) F::run(ly, lds, lds + N * T, t, i, o);
     else F::run(ly, lds, tw, t, i, o);
   };
   // the transform input of slot qs from the loaded b values (zero where the
@@ -232,6 +236,7 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
       }
     });
   }
+  if constexpr (TWL) tw_to_lds<W, N>(tw, lds + N * T, tid, T * P);  // after the data loads
   LI lay{p};
   fft(in, out, lay);
   auto store = [&](int comp) {
@@ -369,6 +374,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
     load_field(0, nx);
     F::load_tw(tw, t, twr);
   }
+  // FP32: the twiddles from a copy of the table in LDS behind the pencils
+  // (tw_to_lds; the first stage's barrier makes it visible)
+  constexpr bool TWL = sizeof(W) == 4 && GT_TWLDS32;
+  const cx<W>* twt = TWL ? lds + N * T : tw;
 #pragma unroll
   for (int f = 0; f < 3; f++) {
     cx<W> in[1][M0 * R0], o[1][ML * RL];
@@ -382,9 +391,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
       F::run(lay, lds, tw, twr, t, in, o);
     } else {
       load_field(f, nx);
+      if constexpr (TWL) {
+        if (f == 0) tw_to_lds<W, N>(tw, lds + N * T, tid, T * P);  // after the first field's loads
+      }
 #pragma unroll
       for (int q = 0; q < M0 * R0; q++) in[0][q] = nx[q];
-      F::run(lay, lds, tw, t, in, o);
+      F::run(lay, lds, twt, t, in, o);
     }
 #pragma unroll
     for (int q = 0; q < ML * RL; q++) y[f][q] = o[0][q];
@@ -509,6 +521,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// dynamic LDS of the z passes: the transform's rows, and for FP32 the copy
+// of the twiddle table behind them
+template <typename W, int N, int T>
+constexpr size_t zlds() {
+  return (size_t)N * T * sizeof(cx<W>) + ((sizeof(W) == 4 && GT_TWLDS32) ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
+}
 template <typename W, int N, int T>
 static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, const void* tw, hipStream_t s) {
   constexpr int P = pencil_P(N);
@@ -521,13 +539,13 @@ static hipError_t zinv_nt(int nout, const KP& kp, const void* b, void* work, con
   const cx<W>* t = (const cx<W>*)tw;
   if constexpr (pz_len(N)) {
     if (kp.pruned && pz_cut(N, kp.kzcut)) {
-      if (pair) k_zinv<W, N, T, P, false, true, GT_ZINV_PAIR != 0><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
-      else k_zinv<W, N, T, P, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
+      if (pair) k_zinv<W, N, T, P, false, true, GT_ZINV_PAIR != 0><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, bb, w, t, nb);
+      else k_zinv<W, N, T, P, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, bb, w, t, nb);
       return hipGetLastError();
     }
   }
-  if (pair) k_zinv<W, N, T, P, false, false, GT_ZINV_PAIR != 0><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
-  else k_zinv<W, N, T, P><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, bb, w, t, nb);
+  if (pair) k_zinv<W, N, T, P, false, false, GT_ZINV_PAIR != 0><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, bb, w, t, nb);
+  else k_zinv<W, N, T, P><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, bb, w, t, nb);
   return hipGetLastError();
 }
 
@@ -553,26 +571,26 @@ static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b
     // the step epilogues of the pruned step: dropped output slots
     if (kp.pruned && pz_cut(N, kp.kzcut) && (epi == EPI_STEP1 || epi == EPI_STEP2)) {
       if (epi == EPI_STEP1) {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
       } else {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
       }
       return hipGetLastError();
     }
   }
   switch (epi) {
     case EPI_STEP1:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
       break;
     case EPI_STEP2:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
       break;
-    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t); break;
-    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, N * T * sizeof(cx<W>), s>>>(kp, w, bb, o, t); break;
+    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t); break;
+    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -162,9 +162,10 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
 
 /* Launch-shape autotuner (no counterpart in the reference): times nrep
  * evaluations of the nonlinear term of the current state for each pencil
- * width (8 or 16 pencils per block) of the z-inverse, z-forward and y
- * passes and keeps the fastest, then the z chunk below.  Results do not
- * depend on the widths.
+ * width (8 or 16 pencils per block) of the z-inverse and z-forward passes
+ * and keeps the fastest of each, then the width of the y passes together
+ * with the z chunk below, by whole-evaluation time.  Results do not depend
+ * on the widths.
  * gt_set_tiles forces a triple (0 = default); gt_get_tiles reports it and
  * the tuned time per evaluation in ms (0 before tuning). */
 int gt_autotune(gt_ctx* c, int nrep);
@@ -174,7 +175,7 @@ int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
  * run in chunks of `planes` z planes (0: all at once), so that each chunk's
  * intermediates are read back while they sit in the device's memory-side
  * cache; with with_d = 0 the y-forward pass runs once after the chunks.
- * gt_autotune picks both (after the widths, from whole-evaluation times;
+ * gt_autotune picks both, with the y width (from whole-evaluation times;
  * gt_get_tiles then reports that time).  Results do not depend on them.
  * with_d may be NULL in gt_get_zchunk.  No counterpart in the reference. */
 int gt_set_zchunk(gt_ctx* c, int planes, int with_d);
