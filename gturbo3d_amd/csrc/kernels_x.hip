@@ -44,6 +44,11 @@ namespace gt {
 #ifndef GT_X_PRETW
 #define GT_X_PRETW 1
 #endif
+// FP32 work fields in the three-wave kernel (twiddles from an LDS copy of
+// the table) instead of the six-wave one
+#ifndef GT_X3_FP32
+#define GT_X3_FP32 0
+#endif
 // FP32 six-wave kernel: twiddles from an LDS copy of the table (0: from the
 // global table, measured 2.5% faster at 512^3)
 #ifndef GT_X_TWLDS32
@@ -211,7 +216,14 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
     cx<W> iu[1][FI::E0], iw[1][FI::E0];
     load(work + c * kp.fs, iu);
     load(work + (3 + c) * kp.fs, iw);
-    if constexpr (GT_X_PRETW) {
+    if constexpr (sizeof(W) == 4) {
+      // FP32 (GT_X3_FP32): the twiddles from an LDS copy of the table
+      tw_to_lds<W, N>(tw, lds + 3 * N, tid, 3 * P);
+      __syncthreads();
+      FI::run(lay, lds, lds + 3 * N, t, iu, uo);
+      stage_sync<SY>();
+      FI::run(lay, lds, lds + 3 * N, t, iw, wo);
+    } else if constexpr (GT_X_PRETW) {
       // the twiddles of all three transforms, loaded with the data
       FI::load_tw(tw, t, twr);
       FI::run(lay, lds, tw, twr, t, iu, uo);
@@ -265,7 +277,8 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   __syncthreads();  // every w row has been read before the rows are reused
   // forward transform of P_{c+2} in row c, then its two rows unpacked
   cx<W> fout[1][FF::EL];
-  if constexpr (GT_X_PRETW) FF::run(lay, lds, tw, twr, t, pr, fout);
+  if constexpr (sizeof(W) == 4) FF::run(lay, lds, lds + 3 * N, t, pr, fout);
+  else if constexpr (GT_X_PRETW) FF::run(lay, lds, tw, twr, t, pr, fout);
   else FF::run(lay, lds, tw, t, pr, fout);
   stage_sync<SY>();
   static_for<0, FF::ML * FF::RL>([&](auto qc) {
@@ -403,7 +416,7 @@ __global__ void k_forcing_seq(cx<double>* b, long long fs, const FEnt* __restric
 template <typename W, int N>
 constexpr bool use_x3() {
   using PL = Plan<N>;
-  return sizeof(W) == 8 && row_P(N) == 64 && PL::radix(0) == PL::radix(PL::S - 1) &&
+  return (sizeof(W) == 8 || GT_X3_FP32) && row_P(N) == 64 && PL::radix(0) == PL::radix(PL::S - 1) &&
          cdiv(N / PL::radix(0), 64) == cdiv(N / PL::radix(PL::S - 1), 64);
 }
 
@@ -412,7 +425,7 @@ static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t
   constexpr int P = row_P(N);
   dim3 grid(kp.ny / 2, kp.zn, kp.nmem);
   if constexpr (use_x3<W, N>()) {
-    const size_t lds3 = 3 * N * sizeof(cx<W>);
+    const size_t lds3 = 3 * N * sizeof(cx<W>) + (sizeof(W) == 4 ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
     if (pz_cut(N, kp.kxcut))
       k_xfused3<W, N, P, true><<<grid, 3 * P, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
     else
