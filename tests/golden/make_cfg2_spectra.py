@@ -12,6 +12,12 @@ step 400, identically in the oracle and on the GPU).
     python tests/golden/make_cfg2_spectra.py [--steps 2000] [--threads 6]
 
 Writes tests/golden/cfg2_spectra_256.npz.  Takes about 1.8 h on 6 cores.
+
+With --mode reference the same run uses the reference's stage 1 (dt2 = 0)
+for 300 steps, records every 100, and writes
+tests/golden/cfg2_reference_256.npz (about 16 minutes on 8 cores):
+
+    python tests/golden/make_cfg2_spectra.py --mode reference
 """
 import argparse
 import math
@@ -31,23 +37,30 @@ from gturbo3d_amd.params import FORCING_2D2C, Params  # noqa: E402
 import oracle_lib as ol  # noqa: E402
 
 
-def cfg2_params(n=256):
-    # famp = sqrt(0.1 / nf) with nf = 10 forced modes on (3, 4]
+def cfg2_params(n=256, dt2_mode=1):
+    # famp = sqrt(0.1 / nf) with nf = 10 forced modes on (3, 4]; dt2_mode 1:
+    # midpoint stage 1, 0: the reference's (dt2 = 0)
     return Params(nx=n, ny=n, nz=n, nu=2e-4, alpnu=1.0, mu=0.0, alpmu=0.0, dt=2e-3, omega=5.0,
                   rtrunc=2.0 / 3.0, truncate=1, famp=math.sqrt(0.1 / 10), k1f=3.0, k2f=4.0,
-                  variant=FORCING_2D2C, dt2_mode=1, npas=2000, imix=100)
+                  variant=FORCING_2D2C, dt2_mode=dt2_mode, npas=2000, imix=100)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--mode", choices=["midpoint", "reference"], default="midpoint")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--imix", type=int, default=100)
     ap.add_argument("--threads", type=int, default=6)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--out", default=os.path.join(HERE, "cfg2_spectra_256.npz"))
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
+    ref = a.mode == "reference"
+    if a.steps is None:
+        a.steps = 300 if ref else 2000
+    if a.out is None:
+        a.out = os.path.join(HERE, "cfg2_reference_256.npz" if ref else "cfg2_spectra_256.npz")
     ol.set_threads(a.threads)
-    p = cfg2_params(a.n)
+    p = cfg2_params(a.n, dt2_mode=0 if ref else 1)
     o = ol.Oracle(p)
     assert o.nf == 10, o.nf
     b = initial.initial_state(p, seed=0)
@@ -63,7 +76,7 @@ def main():
         steps.append(s)
         print(f"step {s}: E = {sp.sum():.6f}  ({time.time() - t0:.0f} s)", flush=True)
     np.savez_compressed(a.out, steps=np.array(steps), sp=np.array(sps), sp2d=np.array(sp2ds),
-                        glob=np.array(globs), seed=seed, n=a.n)
+                        glob=np.array(globs), seed=seed, n=a.n, dt2_mode=p.dt2_mode)
     print("wrote", a.out)
 
 

@@ -6,6 +6,11 @@ by tests/golden/make_cfg2_spectra.py).
 Bar: time-averaged energy and kz = 0 spectra within 1% on every shell that
 holds energy, energy and dissipation (from glob) within 1% at every record,
 and the forcing seed after 2000 steps equal.
+
+The same run in the reference's stage-1 mode (dt2 = 0) diverges between
+steps 300 and 400 in both codes; over its first 300 steps (records every
+100, tests/golden/cfg2_reference_256.npz) the spectra and glob of every
+record must match the oracle's to 1e-6.
 """
 import os
 import sys
@@ -22,6 +27,7 @@ from make_cfg2_spectra import cfg2_params  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 FIXTURE = os.path.join(HERE, "golden", "cfg2_spectra_256.npz")
+FIXTURE_REF = os.path.join(HERE, "golden", "cfg2_reference_256.npz")
 
 
 def say(msg):
@@ -60,3 +66,31 @@ def test_long_run_spectra_match_oracle(require_gpu):
         err = np.abs(a - r) / np.abs(r)
         say(f"  {name} at every record: max relative error {err.max():.2e}")
         assert err.max() < 1e-2
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE_REF), reason="reference-mode fixture not generated")
+def test_reference_mode_run_matches_oracle(require_gpu):
+    fx = np.load(FIXTURE_REF)
+    steps = fx["steps"]
+    p = cfg2_params(int(fx["n"]), dt2_mode=int(fx["dt2_mode"]))
+    assert p.dt2_mode == 0
+    say(f"reference-mode run {p.nx}^3, {int(steps[-1])} steps, records every {int(steps[1] - steps[0])}")
+    b = initial.initial_state(p, seed=0)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        seed, done = 0, 0
+        for i, s in enumerate(steps):
+            seed = g.step(int(s) - done, seed)
+            done = int(s)
+            sp, sp2d, _, glob = g.spectra()
+            for name, got, ref in (("energy", sp, fx["sp"][i]), ("kz=0", sp2d, fx["sp2d"][i])):
+                m = ref > 1e-12 * ref.max()
+                err = np.abs(got[m] - ref[m]) / ref[m]
+                say(f"  step {int(s)}: {name} spectrum max relative error {err.max():.2e} over {m.sum()} shells, "
+                    f"E = {got.sum():.6f} (oracle {ref.sum():.6f})")
+                assert err.max() < 1e-6
+            gr = fx["glob"][i]
+            gerr = np.abs(glob - gr) / np.maximum(np.abs(gr), 1e-12 * np.abs(gr).max())
+            say(f"  step {int(s)}: glob max relative error {gerr.max():.2e}")
+            assert gerr.max() < 1e-6
+    assert seed == int(fx["seed"])
