@@ -372,6 +372,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
     ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk[,with_d]]: force the pencil widths (and the z chunk of passes B-D) "
                          "instead of autotuning")
+    ap.add_argument("--fusion", choices=["auto", "on", "off"], default="auto",
+                    help="the z-forward passes fused with the z-inverse passes that follow them where compiled "
+                         "(auto: the library's default, on in FP64 and off in FP32-bulk)")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
     ap.add_argument("--legs", default=None,
                     help="step mode: comma-separated legs, the first giving the line's value (default "
@@ -425,7 +428,7 @@ def main():
             key = leg.replace("-", "_")
             line[key] = {k: sub[k] for k in ("value", "unit", "ms_per_step", "dtype", "alg_bytes_per_step",
                                               "alg_hbm_frac", "pass_ms_per_step", "roofline", "state_finite")}
-            line[key]["config"] = {"tiles": sub["config"]["tiles"], "pruned": sub["config"]["pruned"]}
+            line[key]["config"] = {k: sub["config"][k] for k in ("tiles", "pruned", "fused_stage_boundary")}
             line[key]["speedup_vs_" + legs[0].replace("-", "_")] = line["ms_per_step"] / sub["ms_per_step"]
         elif leg == "adjoint":
             line["adjoint"] = adjoint_leg(args)
@@ -451,6 +454,8 @@ def step_leg(args, precision, with_cpu):
 
     torch.cuda.init()
     ctx = lib.Context(p)
+    if args.fusion != "auto":
+        ctx.set_fusion(args.fusion == "on")
     ctx.set_state(b)
     tiles = None
     if args.tiles:
@@ -488,6 +493,7 @@ def step_leg(args, precision, with_cpu):
     ms, launches, bytes_per = ctx.prof_read()
     ctx.prof_enable(False)
     pruned = ctx.pruned()
+    fused = ctx.fusion()[1]
     out = ctx.get_state()
     ctx.close()
     finite = all(np.isfinite(f).all() for f in out)
@@ -572,6 +578,7 @@ def step_leg(args, precision, with_cpu):
                         f"dt=1e-3, {args.precision}",
             "grid": [n, n, n],
             "pruned": pruned,
+            "fused_stage_boundary": fused,
             "tiles": tiles,
         },
         "hbm_frac_dense_model": dense_bytes * args.steps / t / (HBM_PEAK_GBS * 1e9),

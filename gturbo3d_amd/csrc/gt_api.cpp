@@ -31,6 +31,9 @@ hipError_t launch_diag(const KP& kp, const void* b, const void* nl, double* part
                        int nbin, hipStream_t s);
 int diag_len(int nbin);
 hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* tw, hipStream_t s);
+bool zfwd_zinv_supported(const KP& kp);
+hipError_t launch_zfwd_zinv(int prec, int stage, const KP& kp, void* work, const void* b, void* out, const void* tw,
+                            const FAux& fx, hipStream_t s);
 hipError_t launch_zinv_adj(const KP& kp, const void* nu, double s, bool ze, void* out, const void* tw,
                            hipStream_t st);
 hipError_t launch_zfwd_adj(const KP& kp, const void* work, void* out, const void* tw, const void* lam,
@@ -84,6 +87,10 @@ struct gt_ctx {
   // bumped whenever something an ensemble's captured graphs bake in changes:
   // the pruned/dense mode (extents of every launch), the tile widths, the z chunk
   unsigned gen = 0;
+  // the z-forward passes fused with the z-inverse passes that follow them
+  // where compiled (gt_set_fusion); on by default in FP64 (where it is
+  // measured faster), off in FP32-bulk
+  bool fuse_ea = true;
   int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
   // batches: nmem members with their own omega (and forcing seeds)
@@ -120,6 +127,11 @@ struct gt_ctx {
   double* dtab = nullptr;    // fkx fky fkz fkxn fkyn fkzn
   int16_t* dkyext = nullptr;
   int16_t* dkzext = nullptr;
+  // the forcing entries of each column tile (FAux): CSR over ky index *
+  // (nxp / T) + kx tile, T the default z-forward width
+  std::vector<int> fstart, fidx;
+  int* dfstart = nullptr;
+  int* dfidx = nullptr;
   void* tw64[3] = {nullptr, nullptr, nullptr};  // x, y, z
   void* tw32[3] = {nullptr, nullptr, nullptr};
   FEnt* dforce = nullptr;
@@ -436,10 +448,9 @@ struct ProfScope {
     if (e_ != hipSuccess) return fail(GT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// One evaluation of N(a; b) through passes A-D, then the pass-E epilogue.
-int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out, hipStream_t st) {
+// Passes B-D of one evaluation: work fields 0-5 (pass A's output) -> 0-2.
+int nonlinear_mid(gt_ctx* c, const KP& kp, hipStream_t st) {
   const int pr = c->prec;
-  LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), st));
   // B, C and D in chunks of zchunk planes along z: a chunk's intermediates
   // are read by the next pass while they are still in the device's
   // memory-side cache (256 MB on MI355X).  The chunk size is a launch shape,
@@ -455,9 +466,42 @@ int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<do
     if (dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), st));
   }
   if (!dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), st));
-  const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
-  LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, c->b, out, twp(c, 2), st));
   return GT_OK;
+}
+
+// One evaluation of N(a; b) through passes A-D, then the pass-E epilogue.
+// bold: the start-of-step state read by the Coriolis term and the RK update
+// of pass E (c->b when null)
+int nonlinear_stage(gt_ctx* c, const KP& kp, const cx<double>* a, int epi, cx<double>* out, hipStream_t st,
+                    const cx<double>* bold = nullptr) {
+  const int pr = c->prec;
+  LAUNCH(PASS_A, launch_zinv(pr, 6, kp, a, c->work, twp(c, 2), st));
+  if (int rc = nonlinear_mid(c, kp, st)) return rc;
+  const int pid = (epi == EPI_STEP2) ? PASS_E2 : PASS_E1;
+  LAUNCH(pid, launch_zfwd(pr, epi, kp, c->work, bold ? bold : c->b, out, twp(c, 2), st));
+  return GT_OK;
+}
+
+// Both stages of one step from bin into bout (which may be bin).  With
+// fusion, stage 1's pass E leaves bh in registers and runs stage 2's pass A
+// on it, so bh is neither stored nor read back.
+int step_stages(gt_ctx* c, const KP& kp, const cx<double>* bin, cx<double>* bout, hipStream_t st) {
+  if (c->fuse_ea && zfwd_zinv_supported(kp)) {
+    const int pr = c->prec;
+    // stage 1: passes A-D of N1 = nlt(b; b)                 (src/main.f90:124-131)
+    LAUNCH(PASS_A, launch_zinv(pr, 6, kp, bin, c->work, twp(c, 2), st));
+    if (int rc = nonlinear_mid(c, kp, st)) return rc;
+    // bh = emk (b + dt2 N1) (src/main.f90:141), then pass A of nlt(bh; b)
+    LAUNCH(PASS_EA, launch_zfwd_zinv(pr, 1, kp, c->work, bin, nullptr, twp(c, 2), FAux{}, st));
+    // stage 2: passes B-E, b = emk (emk b + dt N2)          (src/main.f90:144-146)
+    if (int rc = nonlinear_mid(c, kp, st)) return rc;
+    LAUNCH(PASS_E2, launch_zfwd(pr, EPI_STEP2, kp, c->work, bin, bout, twp(c, 2), st));
+    return GT_OK;
+  }
+  // stage 1: N1 = nlt(b; b), bh = emk (b + dt2 N1)      (src/main.f90:124-141)
+  if (int rc = nonlinear_stage(c, kp, bin, EPI_STEP1, c->bh, st, bin)) return rc;
+  // stage 2: N2 = nlt(bh; b), b = emk (emk b + dt N2)   (src/main.f90:144-146)
+  return nonlinear_stage(c, kp, c->bh, EPI_STEP2, bout, st, bin);
 }
 
 // forcing increments of one step (src/forcing_3d3c.f90:26-72,
@@ -573,6 +617,8 @@ void free_ctx(gt_ctx* c) {
   hipFree(c->dtab);
   hipFree(c->dkyext);
   hipFree(c->dkzext);
+  hipFree(c->dfstart);
+  hipFree(c->dfidx);
   for (int a = 0; a < 3; a++) {
     hipFree(c->tw64[a]);
     hipFree(c->tw32[a]);
@@ -657,6 +703,7 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
   c->nxp = (c->nxh + 15) / 16 * 16;
   c->fs = (long long)c->nz * c->ny * c->nxp;
   c->prec = p->precision;
+  c->fuse_ea = c->prec == GT_PREC_FP64;
   c->nmem = nmem;
   c->omegas.assign(nmem, p->omega);
   if (omegas)
@@ -679,6 +726,20 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
                                                         : ((f.i == 1 && f.j == 1) || f.i == 1 || f.i == c->nxh);
       if (conj) offs.push_back(elem_off(c, f.i - 1, c->jc[f.j - 1] - 1, c->kc[f.k - 1] - 1));
     }
+    // entries by column tile, in table order within a tile (the order
+    // forcing_entries emits them)
+    const int T = pencil_T(c->nz), ntx = c->nxp / T, zs = c->ny * c->nxp;
+    std::vector<int> key(offs.size());
+    c->fstart.assign((size_t)c->ny * ntx + 1, 0);
+    for (size_t e = 0; e < offs.size(); e++) {
+      const long long r = offs[e] % zs;
+      key[e] = (int)(r / c->nxp) * ntx + (int)(r % c->nxp) / T;
+      c->fstart[key[e] + 1]++;
+    }
+    for (size_t k = 1; k < c->fstart.size(); k++) c->fstart[k] += c->fstart[k - 1];
+    c->fidx.assign(offs.size(), 0);
+    std::vector<int> fill(c->fstart.begin(), c->fstart.end() - 1);
+    for (size_t e = 0; e < offs.size(); e++) c->fidx[fill[key[e]]++] = (int)e;
     std::sort(offs.begin(), offs.end());
     c->force_seq = std::adjacent_find(offs.begin(), offs.end()) != offs.end();
   }
@@ -713,6 +774,12 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
   if (rc == GT_OK) chk(hipMalloc(&c->dkyext, sizeof(int16_t) * c->kyext.size()), "extents");
   if (rc == GT_OK)
     chk(hipMemcpy(c->dkyext, c->kyext.data(), sizeof(int16_t) * c->kyext.size(), hipMemcpyHostToDevice), "extents");
+  if (rc == GT_OK) chk(hipMalloc(&c->dfstart, sizeof(int) * c->fstart.size()), "forcing tiles");
+  if (rc == GT_OK)
+    chk(hipMemcpy(c->dfstart, c->fstart.data(), sizeof(int) * c->fstart.size(), hipMemcpyHostToDevice), "forcing tiles");
+  if (rc == GT_OK) chk(hipMalloc(&c->dfidx, sizeof(int) * std::max<size_t>(c->fidx.size(), 1)), "forcing tiles");
+  if (rc == GT_OK && !c->fidx.empty())
+    chk(hipMemcpy(c->dfidx, c->fidx.data(), sizeof(int) * c->fidx.size(), hipMemcpyHostToDevice), "forcing tiles");
   if (rc == GT_OK) chk(hipMalloc(&c->dkzext, sizeof(int16_t) * c->kzext.size()), "extents");
   if (rc == GT_OK)
     chk(hipMemcpy(c->dkzext, c->kzext.data(), sizeof(int16_t) * c->kzext.size(), hipMemcpyHostToDevice), "extents");
@@ -832,15 +899,50 @@ int gt_get_state(gt_ctx* c, double* bx, double* by, double* bz) { return get_sta
 static int enqueue_steps(gt_ctx* c, hipStream_t st, int nsteps, const FEnt* dforce) {
   const size_t per = entries_per_step(c);
   const KP kp = make_kp(c, false);
+  const bool fuse = c->fuse_ea && zfwd_zinv_supported(kp);
+  const int pr = c->prec;
   for (int s = 0; s < nsteps; s++) {
-    // stage 1: N1 = nlt(b; b), bh = emk (b + dt2 N1)      (src/main.f90:124-141)
-    if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, st)) return rc;
-    // stage 2: N2 = nlt(bh; b), b = emk (emk b + dt N2)   (src/main.f90:144-146)
-    if (int rc = nonlinear_stage(c, kp, c->bh, EPI_STEP2, c->b, st)) return rc;
-    // forcing (src/main.f90:149)
-    if (per > 0)
-      LAUNCH(PASS_F, launch_forcing(c->b, c->fs, dforce + per * c->nmem * s, (int)(per * c->nmem), c->force_seq, st));
+    const FEnt* fstep = dforce + per * c->nmem * s;
+    if (!fuse) {
+      if (int rc = step_stages(c, kp, c->b, c->b, st)) return rc;
+      // forcing (src/main.f90:149)
+      if (per > 0) LAUNCH(PASS_F, launch_forcing(c->b, c->fs, fstep, (int)(per * c->nmem), c->force_seq, st));
+      continue;
+    }
+    // fused: pass A of stage 1 was run by the previous step's last kernel,
+    // except in the first step of the call
+    if (s == 0) LAUNCH(PASS_A, launch_zinv(pr, 6, kp, c->b, c->work, twp(c, 2), st));
+    // stage 1: passes B-D of N1 = nlt(b; b); bh = emk (b + dt2 N1) with
+    // pass A of nlt(bh; b)                       (src/main.f90:124-141)
+    if (int rc = nonlinear_mid(c, kp, st)) return rc;
+    LAUNCH(PASS_EA, launch_zfwd_zinv(pr, 1, kp, c->work, c->b, nullptr, twp(c, 2), FAux{}, st));
+    // stage 2: passes B-E, b = emk (emk b + dt N2) (src/main.f90:144-146)
+    if (int rc = nonlinear_mid(c, kp, st)) return rc;
+    if (s + 1 < nsteps) {
+      // with the forcing (src/main.f90:149) and pass A of the next step
+      FAux fx;
+      fx.ent = fstep;
+      fx.per = (int)per;
+      fx.start = c->dfstart;
+      fx.idx = c->dfidx;
+      LAUNCH(PASS_EF, launch_zfwd_zinv(pr, 2, kp, c->work, c->b, c->b, twp(c, 2), fx, st));
+    } else {
+      LAUNCH(PASS_E2, launch_zfwd(pr, EPI_STEP2, kp, c->work, c->b, c->b, twp(c, 2), st));
+      if (per > 0) LAUNCH(PASS_F, launch_forcing(c->b, c->fs, fstep, (int)(per * c->nmem), c->force_seq, st));
+    }
   }
+  return GT_OK;
+}
+
+// One step from bin into bout (single member, bout != bin): the same kernels
+// as enqueue_steps, with the start-of-step state read from bin.  In pruned
+// mode pass E stores only the kept modes, so bout must already vanish
+// outside them.
+static int enqueue_step_to(gt_ctx* c, hipStream_t st, const cx<double>* bin, cx<double>* bout, const FEnt* dforce) {
+  const size_t per = entries_per_step(c);
+  const KP kp = make_kp(c, false);
+  if (int rc = step_stages(c, kp, bin, bout, st)) return rc;
+  if (per > 0) LAUNCH(PASS_F, launch_forcing(bout, c->fs, dforce, (int)per, c->force_seq, st));
   return GT_OK;
 }
 
@@ -1220,7 +1322,10 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
   by[PASS_D] = full * 3 * cw + colsY * c->nz * 3 * cw;
   by[PASS_E1] = colsA * c->nz * 3 * cw + keepE * 3 * cb + keepE * 3 * cb;
   by[PASS_E2] = by[PASS_E1];
+  by[PASS_EA] = colsA * c->nz * 3 * cw + keepE * 3 * cb + colsA * c->nz * 6 * cw;
   by[PASS_F] = (double)entries_per_step(c) * (sizeof(FEnt) + 6 * cb);
+  // the new b stored too, and the step's forcing entries read
+  by[PASS_EF] = by[PASS_EA] + keepE * 3 * cb + (double)entries_per_step(c) * sizeof(FEnt);
   // passes B-D launch once per z chunk: the mean bytes of one launch
   if (c->zchunk > 0 && c->zchunk < c->nz) {
     const double nch = (double)((c->nz + c->zchunk - 1) / c->zchunk);
@@ -1261,6 +1366,21 @@ int gt_set_zchunk(gt_ctx* c, int planes, int with_d) {
   c->zchunk = (planes >= c->nz) ? 0 : planes;
   c->zchunk_d = with_d != 0;
   c->gen++;
+  return GT_OK;
+}
+
+int gt_set_fusion(gt_ctx* c, int on) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->fuse_ea != (on != 0)) c->gen++;
+  c->fuse_ea = on != 0;
+  return GT_OK;
+}
+
+int gt_get_fusion(gt_ctx* c, int* on, int* active) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!on) return fail(GT_EINVAL, "gt_get_fusion: null");
+  *on = c->fuse_ea ? 1 : 0;
+  if (active) *active = (c->fuse_ea && zfwd_zinv_supported(make_kp(c, false))) ? 1 : 0;
   return GT_OK;
 }
 
@@ -1339,6 +1459,13 @@ int gt_autotune(gt_ctx* c, int nrep) {
   };
   c->tz = pick(tA, c->nz);
   c->tze = pick(tE, c->nz);
+  // the fused stage-1 z-forward pass (gt_set_fusion) is compiled at the
+  // default width only: with fusion on, the width that allows it is kept
+  if (c->fuse_ea) {
+    const int te = c->tze;
+    c->tze = 0;
+    if (!zfwd_zinv_supported(make_kp(c, false))) c->tze = te;
+  }
   // width of the y passes and z chunk of passes B-D together, by the wall
   // time of whole evaluations: the y passes share the memory-side cache with
   // the x pass chunk by chunk, so their own event times (taken unchunked)
@@ -1726,8 +1853,7 @@ int step_adjoint(gt_ctx* c, const KP& kp, const cx<double>* bn, const cx<double>
   cx<double>* h = A.vec + 5 * f3;
   cx<double>* g1 = A.vec + 6 * f3;
   HIPCHK(launch_adj_ze(kp, lam_in, nu, st));
-  HIPCHK(hipMemcpyAsync(c->b, bn, sizeof(cx<double>) * f3, hipMemcpyDeviceToDevice, st));
-  if (int rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, bh, st)) return rc;
+  if (int rc = nonlinear_stage(c, kp, bn, EPI_STEP1, bh, st, bn)) return rc;
   HIPCHK(launch_zinv(0, 6, kp, bh, A.work, c->tw64[2], st));
   if (int rc = adjoint_nlt_tail(c, kp, nu, c->p.dt, false, g, nullptr)) return rc;
   HIPCHK(launch_adj_post(kp, nu, g, lam_out, st));
@@ -1816,6 +1942,9 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   const int K = A.every;
   HIPCHK(hipMemsetAsync(A.work, 0, sizeof(cx<double>) * 9 * c->fs, st));
   HIPCHK(hipMemsetAsync(A.vec, 0, 7 * sbytes, st));
+  // the recomputed states are stored by pass E, which in pruned mode writes
+  // only the kept modes: the rest of every slot must be zero
+  if (K > 1) HIPCHK(hipMemsetAsync(A.seg + f3, 0, (size_t)(K - 1) * sbytes, st));
   cx<double>* lam = A.vec;          // the cotangent of the current state
   cx<double>* lam_next = A.vec + f3;
   const double* src[3] = {lx, ly, lz};
@@ -1830,19 +1959,18 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
     int32_t sd = A.seeds[s];
     for (int i = 0; i < len; i++) forcing_entries(c, &sd, per ? hunit.data() + per * i : nullptr, 0, true);
     if (per) HIPCHK(hipMemcpyAsync(A.dunit, hunit.data(), sizeof(FEnt) * per * len, hipMemcpyHostToDevice, st));
-    // recompute the segment's states b_{sK} .. b_{sK+len-1}
+    // recompute the segment's states b_{sK} .. b_{sK+len-1}: the first is
+    // the checkpoint itself, each next one is stepped out of place into its
+    // own slot (slot 0 of A.seg is unused)
     int32_t sr = A.seeds[s];
-    HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)s * f3, sbytes, hipMemcpyDeviceToDevice, st));
     if (len > 1)
       if (int rc = stage_forcing(c, len - 1, &sr)) return rc;
-    for (int i = 0; i < len; i++) {
-      HIPCHK(hipMemcpyAsync(A.seg + (size_t)i * f3, c->b, sbytes, hipMemcpyDeviceToDevice, st));
-      if (i + 1 < len)
-        if (int rc = enqueue_steps(c, st, 1, c->dforce + per * i)) return rc;
-    }
+    auto state = [&](int i) -> cx<double>* { return i == 0 ? A.ckpt + (size_t)s * f3 : A.seg + (size_t)i * f3; };
+    for (int i = 0; i + 1 < len; i++)
+      if (int rc = enqueue_step_to(c, st, state(i), state(i + 1), c->dforce + per * i)) return rc;
     for (int i = len - 1; i >= 0; i--) {
       if (per) HIPCHK(launch_adj_fdot(lam, c->fs, A.dunit + per * i, (int)per, A.ddots + per * i, st));
-      if (int rc = step_adjoint(c, kp, A.seg + (size_t)i * f3, lam, lam_next)) return rc;
+      if (int rc = step_adjoint(c, kp, state(i), lam, lam_next)) return rc;
       std::swap(lam, lam_next);
     }
     if (per) HIPCHK(hipMemcpyAsync(hdots.data(), A.ddots, sizeof(double) * per * len, hipMemcpyDeviceToHost, st));

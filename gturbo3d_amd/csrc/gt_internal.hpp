@@ -66,6 +66,18 @@ struct FEnt {
   double d[6];    // (re, im) for bx, by, bz
 };
 
+// One step's forcing for the z-forward pass that adds it (the stage-2 pass
+// fused with the next step's z-inverse pass): ent holds the step's entries,
+// per for each member in turn; the entries of column tile key = (ky index)
+// * (nxp / T) + (kx tile) are idx[start[key] .. start[key + 1]), member-
+// relative, in table order.
+struct FAux {
+  const FEnt* ent = nullptr;
+  int per = 0;
+  const int* start = nullptr;
+  const int* idx = nullptr;
+};
+
 // pass identifiers for timing / byte accounting
 enum PassId {
   PASS_A = 0,  // z inverse + curl/k^2 (3 -> 6 fields)
@@ -75,7 +87,9 @@ enum PassId {
   PASS_E1 = 4, // z forward + Coriolis/invcurl/trunc + step1
   PASS_E2 = 5, // z forward + Coriolis/invcurl/trunc + step2
   PASS_F = 6,  // forcing scatter
-  NPASS = 7
+  PASS_EA = 7, // E1 fused with pass A of stage 2 (bh kept in registers)
+  PASS_EF = 8, // E2 + forcing fused with pass A of the next step
+  NPASS = 9
 };
 
 }  // namespace gt

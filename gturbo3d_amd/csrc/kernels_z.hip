@@ -302,14 +302,28 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
 #endif
 // PZ: the output slots that mid_slot() prunes (kzcut < 3N/8) are never
 // stored, so the transform's last stage drops them.
-template <typename W, int N, int T, int P, int EPI, bool VS = false, bool PZ = false>
+// FA = 1 (with EPI_STEP1): the stage-1 state bh is not stored.  It stays
+// in registers, and the block runs pass A of stage 2 on it at once: the six
+// z-inverse transforms of u = curl bh and w = k^2 bh of its column, written
+// to wout (the work fields whose products it has just read; every load of
+// the column precedes the first store, and no other block touches the
+// column).  bh then never goes through HBM.
+// FA = 2 (with EPI_STEP2): the block adds this step's forcing increments of
+// its column (src/main.f90:149) to the new b, stores it, and runs pass A of
+// the next step's stage 1 on it in the same way, so the next step does not
+// read b back.
+// Both need a first radix equal to the last, so that the forward transform
+// leaves every value where the inverse transform takes it.
+template <typename W, int N, int T, int P, int EPI, bool VS = false, bool PZ = false, int FA = 0>
 __global__ __launch_bounds__(T* P)
-__attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE32))) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
-                                               cx<double>* out, const cx<W>* __restrict__ tw) {
+__attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE32))) void k_zfwd(KP kp, const cx<W>* __restrict__ work, const cx<double>* b,
+                                               cx<double>* out, const cx<W>* __restrict__ tw, cx<W>* wout, FAux fx2) {
   // out may be b itself (step2 updates b in place), so neither is restrict
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, -1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
+  static_assert(FA == 0 || ((FA == 1 ? EPI == EPI_STEP1 : EPI == EPI_STEP2) && R0 == RL && M0 == ML),
+                "FA: stage 1 (1) or 2 (2), first radix equal to the last");
   GT_DYN_LDS(cx<W>, lds);  // N T elements
   const int tid = threadIdx.x, p = tid % T, t = tid / T;
   // 1-D grid over columns (kx tile, ky, member).  When a tile row is shorter
@@ -328,8 +342,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   const int tx = col % ntx, j = (col / ntx) % kp.ny, mem = col / (ntx * kp.ny);
   const int kx = tx * T + p;
   work += mem * kp.wstride;
+  if constexpr (FA != 0) wout += mem * kp.wstride;
   if (b) b += mem * kp.bstride;
-  out += mem * kp.bstride;
+  if constexpr (FA != 1) out += mem * kp.bstride;
   const double omega = kp.omega_m ? kp.omega_m[mem] : kp.omega;
   int K = -1;
   if (kx <= kp.kxcut) K = kp.pruned ? (int)kp.kzext[j * kp.nxp + kx] : (N >> 1);
@@ -401,7 +416,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
 #pragma unroll
     for (int q = 0; q < ML * RL; q++) y[f][q] = o[0][q];
   }
-  if (K < 0) return;
+  // with FA every thread takes part in the transforms after the epilogue;
+  // a pencil with K < 0 keeps no slot (keep_idx) and stores nothing
+  if constexpr (FA == 0) {
+    if (K < 0) return;
+  }
 
   const int kxc = kx < kp.nxh ? kx : kp.nxh - 1;
   const double fx = kp.fkx[kxc], fy = kp.fky[j];
@@ -413,6 +432,28 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
   // that is not stored reads the column's kz = 0 element instead of its own
   // (a line the kept slots fetch anyway).
   constexpr int NS = ML * RL;
+  // FA: the new state of every slot (zero where not kept) and the slot's
+  // kz wavenumber
+  cx<double> bhr[FA ? 3 : 1][FA ? NS : 1];
+  double fza[FA ? NS : 1];
+  // FA = 2: this step's forcing entries of the block's column tile,
+  // fx2.idx[fa0 .. fa1) (member-relative entry numbers, in table order)
+  int fa0 = 0, fa1 = 0;
+  const FEnt* fent = nullptr;
+  if constexpr (FA == 2) {
+    const int key = j * (kp.nxp / T) + tx;
+    fa0 = fx2.start[key];
+    fa1 = fx2.start[key + 1];
+    fent = fx2.ent + (long long)mem * fx2.per;
+  }
+  if constexpr (FA) {
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      fza[q] = 0.0;
+#pragma unroll
+      for (int f = 0; f < 3; f++) bhr[f][q] = {0.0, 0.0};
+    }
+  }
   constexpr int EPB0 = TUNED ? GT_ZFWD_EPB64 : sizeof(W) == 8 ? 1 : GT_ZFWD_EPB32;
   constexpr int EPB = (NS % EPB0 != 0) ? NS : EPB0;
   static_for<0, NS / EPB>([&](auto bc) {
@@ -510,12 +551,74 @@ __attribute__((amdgpu_waves_per_eu(sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_ZFWD_WPE3
             if constexpr (EPI == EPI_STEP1) o = {e * (bb.x + kp.dt2 * nl[f].x), e * (bb.y + kp.dt2 * nl[f].y)};
             else o = {e * (e * bb.x + kp.dt * nl[f].x), e * (e * bb.y + kp.dt * nl[f].y)};
             if (k0) o = {0.0, 0.0};
-            st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
+            if constexpr (FA == 2) {
+              // forcing (src/main.f90:149): the increments at this mode, in
+              // table order (a position may repeat)
+              const long long pos = c + lane, mo = (long long)mem * 3 * kp.fs;
+              for (int q = fa0; q < fa1; q++) {
+                const FEnt& fe = fent[fx2.idx[q]];
+                if (fe.off - mo == pos) {
+                  o.x = o.x + fe.d[2 * f];
+                  o.y = o.y + fe.d[2 * f + 1];
+                }
+              }
+            }
+            if constexpr (FA == 1) bhr[f][qs] = o;
+            else st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
+            if constexpr (FA == 2) bhr[f][qs] = o;
           }
+          if constexpr (FA) fza[qs] = fz;
         }
       }
     });
   });
+  if constexpr (FA) {
+    // pass A on the new state (as k_zinv): output q < 3 is u_q = curl_q bh,
+    // u_x = i(ky bz - kz by), u_y = i(kz bx - kx bz), u_z = i(kx by - ky bx)
+    // (src/produ.f90:48-53); q >= 3 is w = k^2 bh of field q - 3
+    // (src/nlt.f90:30-36).  Slot qs holds element t + P m + r N/R of the
+    // forward output, which is the inverse transform's input slot qs.
+    using FI = Fft<W, N, P, 1, +1, LI>;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      cx<W> in[1][M0 * R0], o[1][ML * RL];
+      static_for<0, NS>([&](auto qc) {
+        constexpr int qs = decltype(qc)::value, m = qs / RL, r = qs % RL;
+        if constexpr (PZ && mid_slot<N, P, P * m + r * F::NRL>()) {
+          in[0][qs] = {W(0), W(0)};
+        } else {
+          const double fz = fza[qs];
+          double re, im;
+          if (q < 3) {
+            const int fa = q == 0 ? 2 : q == 1 ? 0 : 1;
+            const int fb = q == 0 ? 1 : q == 1 ? 2 : 0;
+            const double ca = (q == 0) ? fy : (q == 1) ? fz : fx;
+            const double cb = (q == 0) ? fz : (q == 1) ? fx : fy;
+            const cx<double> va = bhr[fa][qs], vb = bhr[fb][qs];
+            re = -ca * va.y + cb * vb.y;
+            im = ca * va.x - cb * vb.x;
+          } else {
+            const double k2 = fx * fx + fy * fy + fz * fz;
+            const cx<double> va = bhr[q - 3][qs];
+            re = k2 * va.x;
+            im = k2 * va.y;
+          }
+          in[0][qs] = {(W)re, (W)im};
+        }
+      });
+      if constexpr (PRE) FI::run(lay, lds, tw, twr, t, in, o);
+      else FI::run(lay, lds, twt, t, in, o);
+      if (K >= 0) {
+        cx<W>* dst = wout + q * kp.fs + tile;
+#pragma unroll
+        for (int m = 0; m < ML; m++)
+          if (F::out_ok(t, m))
+#pragma unroll
+            for (int r = 0; r < RL; r++)
+              st_row_p<NT_A>(dst + (long long)(P * m + r * F::NRL) * zs, lane, o[0][m * RL + r]);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -571,26 +674,26 @@ static hipError_t zfwd_nt(int epi, const KP& kp, const void* work, const void* b
     // the step epilogues of the pruned step: dropped output slots
     if (kp.pruned && pz_cut(N, kp.kzcut) && (epi == EPI_STEP1 || epi == EPI_STEP2)) {
       if (epi == EPI_STEP1) {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
+        else k_zfwd<W, N, T, P, EPI_STEP1, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
       } else {
-        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
-        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+        if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
+        else k_zfwd<W, N, T, P, EPI_STEP2, false, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
       }
       return hipGetLastError();
     }
   }
   switch (epi) {
     case EPI_STEP1:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP1, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
+      else k_zfwd<W, N, T, P, EPI_STEP1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
       break;
     case EPI_STEP2:
-      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
-      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t);
+      if (vs) k_zfwd<W, N, T, P, EPI_STEP2, true><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
+      else k_zfwd<W, N, T, P, EPI_STEP2><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{});
       break;
-    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t); break;
-    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t); break;
+    case EPI_NONLIN: k_zfwd<W, N, T, P, EPI_NONLIN><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{}); break;
+    case EPI_INIT: k_zfwd<W, N, T, P, EPI_INIT><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, w, bb, o, t, nullptr, FAux{}); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -649,6 +752,50 @@ hipError_t launch_zfwd(int prec, int epi, const KP& kp, const void* work, const 
     default:
       return hipErrorInvalidValue;
   }
+}
+
+// Pass E of stage 1 fused with pass A of stage 2, and pass E of stage 2
+// fused with the forcing and pass A of the next step (k_zfwd with FA 1 and
+// 2): they read the products and b and write the six z-inverse outputs into
+// the work fields (FA 2 also the new b).  Compiled for 512-point z transforms with simple viscosity at the
+// default width of 8 pencils (one block of 8 wavefronts per CU at its
+// register count; 4 pencils, two blocks per CU, measured slower: 1.19
+// against 1.03 ms per step in FP64, 1.63 against 0.78 in FP32-bulk).  The
+// z-forward width of the context must be the default one, which pass E of
+// stage 2 uses.
+bool zfwd_zinv_supported(const KP& kp) {
+  return kp.nz == 512 && kp.emx != nullptr && (alt_T(512) == 0 || kp.tze != alt_T(512));
+}
+
+template <typename W>
+static hipError_t zfwd_zinv_w(int stage, const KP& kp, void* work, const void* b, void* out, const void* tw,
+                              const FAux& fx, hipStream_t s) {
+  constexpr int N = 512, T = pencil_T(N), P = pencil_P(N);
+  const int grid = (kp.kxcut + T) / T * kp.ny * kp.nmem;
+  const cx<double>* bb = (const cx<double>*)b;
+  cx<double>* o = (cx<double>*)out;
+  cx<W>* wk = (cx<W>*)work;
+  const cx<W>* t = (const cx<W>*)tw;
+  const bool pz = kp.pruned && pz_cut(N, kp.kzcut);
+  if (stage == 1) {
+    // out is unused (bh stays in registers)
+    if (pz) k_zfwd<W, N, T, P, EPI_STEP1, true, true, 1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, wk, bb, nullptr, t, wk, fx);
+    else k_zfwd<W, N, T, P, EPI_STEP1, true, false, 1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, wk, bb, nullptr, t, wk, fx);
+  } else {
+    if (pz) k_zfwd<W, N, T, P, EPI_STEP2, true, true, 2><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, wk, bb, o, t, wk, fx);
+    else k_zfwd<W, N, T, P, EPI_STEP2, true, false, 2><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, wk, bb, o, t, wk, fx);
+  }
+  return hipGetLastError();
+}
+
+// stage 1: pass E of stage 1 with pass A of stage 2 (b read, out unused);
+// stage 2: pass E of stage 2 into out (b in place), the forcing of fx, then
+// pass A of the next step
+hipError_t launch_zfwd_zinv(int prec, int stage, const KP& kp, void* work, const void* b, void* out, const void* tw,
+                            const FAux& fx, hipStream_t s) {
+  if (!zfwd_zinv_supported(kp) || (stage != 1 && stage != 2)) return hipErrorInvalidValue;
+  return prec ? zfwd_zinv_w<float>(stage, kp, work, b, out, tw, fx, s)
+              : zfwd_zinv_w<double>(stage, kp, work, b, out, tw, fx, s);
 }
 
 }  // namespace gt

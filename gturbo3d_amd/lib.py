@@ -16,8 +16,9 @@ from .params import Params
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("GT_LIB_PATH") or os.path.join(PKG, "libgturbo3d.so")
 
-NPASS = 7
-PASS_NAMES = ["A_zinv_curl_k2", "B_yinv", "C_x_cross", "D_yfwd", "E1_zfwd_step1", "E2_zfwd_step2", "F_forcing"]
+NPASS = 9
+PASS_NAMES = ["A_zinv_curl_k2", "B_yinv", "C_x_cross", "D_yfwd", "E1_zfwd_step1", "E2_zfwd_step2", "F_forcing",
+              "EA_zfwd_step1_zinv", "EF_zfwd_step2_force_zinv"]
 
 # Every exported symbol of include/gturbo3d.h.
 EXPORTS = [
@@ -27,7 +28,7 @@ EXPORTS = [
     "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
-    "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
+    "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_set_fusion", "gt_get_fusion", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
     "gt_check_finite", "gt_spectra_noalloc",
 ]
 
@@ -93,6 +94,8 @@ def load():
     L.gt_autotune.argtypes = [vp, ip]
     L.gt_set_tiles.argtypes = [vp, ip, ip, ip]
     L.gt_set_zchunk.argtypes = [vp, ip, ip]
+    L.gt_set_fusion.argtypes = [vp, ip]
+    L.gt_get_fusion.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     L.gt_get_zchunk.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     L.gt_get_tiles.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip),
                                ctypes.POINTER(ctypes.c_double)]
@@ -273,6 +276,18 @@ class Context:
         """z planes per chunk of the y-inverse and x passes (0: all at once); with_d:
         the y-forward pass runs in the chunks too (else once after them)."""
         _check(load().gt_set_zchunk(self.h, int(planes), 1 if with_d else 0), "gt_set_zchunk")
+
+    def set_fusion(self, on: bool) -> None:
+        """Fuse the z-forward passes with the z-inverse passes that follow them
+        (bh kept in registers; b not read back within one step call) where the
+        library compiles it.  On by default in FP64, off in FP32-bulk."""
+        _check(load().gt_set_fusion(self.h, 1 if on else 0), "gt_set_fusion")
+
+    def fusion(self):
+        """(on, active): the setting, and whether the current context uses it."""
+        v, a = ctypes.c_int(), ctypes.c_int()
+        _check(load().gt_get_fusion(self.h, ctypes.byref(v), ctypes.byref(a)), "gt_get_fusion")
+        return bool(v.value), bool(a.value)
 
     def zchunk(self):
         """(planes, with_d)"""

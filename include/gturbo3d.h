@@ -152,7 +152,9 @@ int gt_pruned(gt_ctx* c, int* pruned);
 /* Per-pass timing with HIP events on the context stream.  Pass ids:
  * 0 A (z inverse + curl, k^2), 1 B (y inverse), 2 C (x C2R, u x w, x R2C),
  * 3 D (y forward), 4 E1 (z forward + step1), 5 E2 (z forward + step2),
- * 6 F (forcing).  gt_prof_read waits for the stream and returns the summed
+ * 6 F (forcing), 7 EA (E1 fused with stage 2's pass A), 8 EF (E2 fused
+ * with the forcing and the next step's pass A; gt_set_fusion).
+ * gt_prof_read waits for the stream and returns the summed
  * milliseconds and launch counts since enable; bytes[] is the algorithmic
  * HBM traffic of one launch of each pass (every element the pass must
  * read or write, once; for passes B-D run in z chunks, the mean over the
@@ -180,6 +182,20 @@ int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
  * with_d may be NULL in gt_get_zchunk.  No counterpart in the reference. */
 int gt_set_zchunk(gt_ctx* c, int planes, int with_d);
 int gt_get_zchunk(gt_ctx* c, int* planes, int* with_d);
+
+/* Cross-stage fusion (no counterpart in the reference; on by default in
+ * FP64, off in FP32-bulk, where the fused kernels measured slower): the
+ * z-forward pass of stage 1 keeps bh = emk (b + dt2 N1) in registers and
+ * runs stage 2's z-inverse pass (curl bh, k^2 bh) on it in the same kernel,
+ * so bh never goes through device memory; within one gt_step call, the
+ * z-forward pass of stage 2 likewise adds the step's forcing, stores b and
+ * runs the next step's z-inverse pass on it, so b is not read back.
+ * Results equal the unfused step to rounding (contraction order).  Compiled for NZ = 512 with simple
+ * viscosity (alpnu = 1, mu = 0) at the default z-forward width, which
+ * gt_autotune keeps while fusion is on; *active (may be NULL) reports
+ * whether the current context uses it. */
+int gt_set_fusion(gt_ctx* c, int on);
+int gt_get_fusion(gt_ctx* c, int* on, int* active);
 
 /* Batches: nmembers independent members on one grid with the same
  * parameters except omega (omegas[m], or p->omega when omegas is NULL) and
