@@ -32,6 +32,12 @@ def to_complex(t: torch.Tensor):
     return [np.ascontiguousarray(c[f]) for f in range(3)]
 
 
+def _fields_of(t: torch.Tensor):
+    """The three complex fields viewing the memory of a CPU float64 tensor
+    (3, NZ, NY, NXH, 2): the library reads or writes them in place."""
+    return [t[f].numpy().view(np.complex128)[..., 0] for f in range(3)]
+
+
 class StepRun(torch.autograd.Function):
     """b_N = run(b_0, amps): nsteps steps on context ``g`` from forcing seed ``seed``.
 
@@ -47,16 +53,21 @@ class StepRun(torch.autograd.Function):
         ctx.seed_out = g.adjoint_forward(int(nsteps), int(seed), int(every))
         ctx.g = g
         ctx.run = g.adjoint_run
-        return to_real(g.get_state()).to(b0.device)
+        out = torch.empty((3,) + tuple(g.shape) + (2,), dtype=torch.float64)
+        g.get_state(out=_fields_of(out))
+        return out.to(b0.device)
 
     @staticmethod
     def backward(ctx, grad):
         if ctx.g.adjoint_run != ctx.run:
             raise RuntimeError(f"StepRun.backward: the context's checkpoints belong to run {ctx.g.adjoint_run}, "
                                f"not to this node's run {ctx.run} (one live run per context)")
-        gb, ga = ctx.g.adjoint_backward(to_complex(grad))
+        # the library writes dJ/db_0 straight into the result tensor's memory
+        # (its real view is the three complex fields back to back)
+        res = torch.empty(grad.shape, dtype=torch.float64)
+        _, ga = ctx.g.adjoint_backward(to_complex(grad), out=_fields_of(res))
         dev = grad.device
-        return to_real(gb).to(dev), torch.from_numpy(ga.copy()).to(dev), None, None, None, None
+        return res.to(dev), torch.from_numpy(ga.copy()).to(dev), None, None, None, None
 
 
 def run(g: lib.Context, b0: torch.Tensor, amps: torch.Tensor, nsteps: int, seed: int, every: int = 20):

@@ -50,6 +50,23 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 #ifndef GT_ZFWD_EPB64
 #define GT_ZFWD_EPB64 4
 #endif
+// FA: the slots' kz wavenumbers kept in registers from the epilogue (1) or
+// read from the table again for the z-inverse transforms (0), and the output
+// slots per load batch of the epilogue, for FA = 1 and FA = 2.  Measured at
+// 512^3 FP64: FA = 1 with 0 and 2 runs without spills, 5% faster than with
+// 1 and 4 (11 spilled registers); FA = 2 is 4% slower with 0.
+#ifndef GT_EA_FZREG
+#define GT_EA_FZREG 0
+#endif
+#ifndef GT_EA_EPB
+#define GT_EA_EPB 2
+#endif
+#ifndef GT_EF_FZREG
+#define GT_EF_FZREG 1
+#endif
+#ifndef GT_EF_EPB
+#define GT_EF_EPB 4
+#endif
 
 // Pass A: one output component q per block (0..2: u = curl b,
 // 3..5: w = k^2 b), so a block holds only the one or two fields of b it
@@ -434,8 +451,9 @@ __attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_
   constexpr int NS = ML * RL;
   // FA: the new state of every slot (zero where not kept) and the slot's
   // kz wavenumber
+  constexpr bool FZR = FA == 1 ? GT_EA_FZREG : FA == 2 ? GT_EF_FZREG : 0;
   cx<double> bhr[FA ? 3 : 1][FA ? NS : 1];
-  double fza[FA ? NS : 1];
+  double fza[FZR ? NS : 1];
   // FA = 2: this step's forcing entries of the block's column tile,
   // fx2.idx[fa0 .. fa1) (member-relative entry numbers, in table order)
   int fa0 = 0, fa1 = 0;
@@ -449,12 +467,13 @@ __attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_
   if constexpr (FA) {
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-      fza[q] = 0.0;
+      if constexpr (FZR) fza[q] = 0.0;
 #pragma unroll
       for (int f = 0; f < 3; f++) bhr[f][q] = {0.0, 0.0};
     }
   }
-  constexpr int EPB0 = TUNED ? GT_ZFWD_EPB64 : sizeof(W) == 8 ? 1 : GT_ZFWD_EPB32;
+  constexpr int EPB0 = TUNED ? (FA == 1 ? GT_EA_EPB : FA == 2 ? GT_EF_EPB : GT_ZFWD_EPB64)
+                             : sizeof(W) == 8 ? 1 : GT_ZFWD_EPB32;
   constexpr int EPB = (NS % EPB0 != 0) ? NS : EPB0;
   static_for<0, NS / EPB>([&](auto bc) {
     constexpr int q0 = decltype(bc)::value * EPB;
@@ -567,7 +586,7 @@ __attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_
             else st_row_p<NT_E>(out + f * kp.fs + c, lane, o);
             if constexpr (FA == 2) bhr[f][qs] = o;
           }
-          if constexpr (FA) fza[qs] = fz;
+          if constexpr (FZR) fza[qs] = fz;
         }
       }
     });
@@ -587,7 +606,7 @@ __attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_
         if constexpr (PZ && mid_slot<N, P, P * m + r * F::NRL>()) {
           in[0][qs] = {W(0), W(0)};
         } else {
-          const double fz = fza[qs];
+          const double fz = FZR ? fza[qs] : kp.fkz[F::out_pos(t, m, r)];
           double re, im;
           if (q < 3) {
             const int fa = q == 0 ? 2 : q == 1 ? 0 : 1;

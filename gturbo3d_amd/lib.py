@@ -198,9 +198,20 @@ class Context:
         b = self._fields(b, "set_state")
         _check(load().gt_set_state(self.h, *[_ptr(x) for x in b]), "gt_set_state")
 
-    def get_state(self):
-        b = [np.empty(self.shape, np.complex128) for _ in range(3)]
+    def get_state(self, out=None):
+        """The three fields; out: three contiguous complex128 arrays of the
+        state's shape to write them into (else new arrays)."""
+        b = self._out_fields(out, "get_state")
         _check(load().gt_get_state(self.h, *[_ptr(x) for x in b]), "gt_get_state")
+        return b
+
+    def _out_fields(self, out, what):
+        if out is None:
+            return [np.empty(self.shape, np.complex128) for _ in range(3)]
+        b = list(out)
+        if len(b) != 3 or any(x.shape != self.shape or x.dtype != np.complex128 or not x.flags.c_contiguous
+                              for x in b):
+            raise ValueError(f"{what}: out must be three contiguous complex128 arrays of shape {self.shape}")
         return b
 
     def step(self, nsteps: int, seed: int) -> int:
@@ -337,11 +348,13 @@ class Context:
         _check(load().gt_adjoint_forward(self.h, int(nsteps), ctypes.byref(s), int(every)), "gt_adjoint_forward")
         return s.value
 
-    def adjoint_backward(self, lam):
+    def adjoint_backward(self, lam, out=None):
         """dJ/db_0 (three complex fields) and dJ/d(shell amplitudes) from
-        lam = dJ/db_N of the last adjoint_forward run."""
+        lam = dJ/db_N of the last adjoint_forward run.  out: three contiguous
+        complex128 arrays of the state's shape to write dJ/db_0 into (else
+        new arrays)."""
         lam = self._fields(lam, "adjoint_backward")
-        g = [np.empty(self.shape, np.complex128) for _ in range(3)]
+        g = self._out_fields(out, "adjoint_backward")
         nsh = len(self.forcing_shells())
         ga = np.zeros(max(nsh, 1))
         _check(load().gt_adjoint_backward(self.h, *[_ptr(x) for x in lam], *[_ptr(x) for x in g], _ptr(ga), nsh),
