@@ -182,3 +182,33 @@ def test_fused_batch_members_equal_solo_runs(require_gpu):
     assert list(seeds) == [ss0, ss1]
     assert all(np.array_equal(x, y) for x, y in zip(m0, s0))
     assert all(np.array_equal(x, y) for x, y in zip(m1, s1))
+
+
+def test_fused_ensemble_graphs_512(require_gpu):
+    """A captured ensemble of one 512^3 batch (two members), two steps per
+    graph: the graphs hold the fused kernels, their forcing entries come from
+    the ensemble's own buffers.  Five steps (two replays and an eager
+    remainder) against solo fused runs of five steps in one call."""
+    p = cfg512()
+    say("fused ensemble 2 x 512^3, chunk 2")
+    oms, seeds = [2.0, 3.5], [31, 32]
+    ics = [initial.initial_state(p, seed=40 + m) for m in range(2)]
+    A = lib.Batch(p, 2, oms)
+    assert A.fusion() == (True, True)
+    for m in range(2):
+        A.set_member(m, ics[m])
+    ens = lib.Ensemble([A], chunk=2)
+    ens.set_seeds(0, seeds)
+    ens.step(5)
+    ens.sync()
+    got_seeds = ens.get_seeds(0)
+    for m in range(2):
+        q = cfg512(omega=oms[m])
+        q.famp = p.famp
+        ref, rs, _ = run(q, ics[m], seeds[m], 5, True)
+        assert got_seeds[m] == rs
+        err = rel_l2(A.get_member(m), ref)
+        say(f"  member {m}: {err:.2e}")
+        assert err < 1e-14
+    ens.close()
+    A.close()
