@@ -180,24 +180,30 @@ void k_xfused(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
 // instead of 6: measured at the 512^3 chunk shape, 0.184 against 0.228 ms
 // per 64 planes, with the same bits.  FP32 work fields keep the six-wave
 // kernel, which is faster there (its passes are bound by VALU issue).
+// Rows shorter than a wavefront (P < 64 threads per transform): each
+// wavefront holds component c of G = 64 / P row pairs, one transform per P
+// lanes, and the block's LDS row of (c, pair g) is c G + g.
 template <typename W, int N, int P, bool PZ>
-__global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
+__global__ __launch_bounds__(3 * 64) void k_xfused3(KP kp, cx<W>* work, const cx<W>* __restrict__ tw) {
   using L = LaySwz<N>;
   constexpr int SY = SYNC_WAVE;
-  static_assert(P == 64, "one wavefront per transform");
+  static_assert(64 % P == 0, "whole transforms in each wavefront");
+  constexpr int G = 64 / P;  // row pairs per block
   using FI = Fft<W, N, P, 1, +1, L, SY>;
   using FF = Fft<W, N, P, 1, -1, L, SY>;
   static_assert(FI::RL == FF::R0 && FI::ML == FF::M0, "inverse outputs sit at the forward inputs");
-  GT_DYN_LDS(cx<W>, lds);  // 3 N elements
+  GT_DYN_LDS(cx<W>, lds);  // 3 G N elements
   const int tid = threadIdx.x, t = tid % P;
-  const int c = __builtin_amdgcn_readfirstlane(tid / P);  // component of this wavefront
+  const int c = __builtin_amdgcn_readfirstlane(tid / 64);  // component of this wavefront
+  const int g = (tid % 64) / P;                              // row pair of this transform
   const int cn = (c + 1) % 3;
-  const int y0 = 2 * blockIdx.x, z = kp.z0 + blockIdx.y;
+  const int y0 = 2 * (blockIdx.x * G + g), z = kp.z0 + blockIdx.y;
   work += blockIdx.z * kp.wstride;
   const int r0 = (z * kp.ny + y0) * kp.nxp;
   const int r1 = r0 + kp.nxp;
   const int kc = kp.kxcut;
-  const L lay{c, 0};
+  const int row = c * G + g, rown = cn * G + g;  // LDS rows of this and the next component
+  const L lay{row, 0};
   auto load = [&](const cx<W>* src, cx<W> (&in)[1][FI::E0]) {
     static_for<0, FI::M0 * FI::R0>([&](auto qc) {
       constexpr int q = decltype(qc)::value, m = q / FI::R0, r = q % FI::R0;
@@ -218,11 +224,11 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
     load(work + (3 + c) * kp.fs, iw);
     if constexpr (sizeof(W) == 4) {
       // FP32 (GT_X3_FP32): the twiddles from an LDS copy of the table
-      tw_to_lds<W, N>(tw, lds + 3 * N, tid, 3 * P);
+      tw_to_lds<W, N>(tw, lds + 3 * G * N, tid, 3 * 64);
       __syncthreads();
-      FI::run(lay, lds, lds + 3 * N, t, iu, uo);
+      FI::run(lay, lds, lds + 3 * G * N, t, iu, uo);
       stage_sync<SY>();
-      FI::run(lay, lds, lds + 3 * N, t, iw, wo);
+      FI::run(lay, lds, lds + 3 * G * N, t, iw, wo);
     } else if constexpr (GT_X_PRETW) {
       // the twiddles of all three transforms, loaded with the data
       FI::load_tw(tw, t, twr);
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   for (int m = 0; m < FI::ML; m++)
     if (FI::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < FI::RL; r++) lds[c * N + FI::out_pos(t, m, r)] = uo[0][m * FI::RL + r];
+      for (int r = 0; r < FI::RL; r++) lds[row * N + FI::out_pos(t, m, r)] = uo[0][m * FI::RL + r];
   __syncthreads();
   cx<W> pr[1][FF::E0];
 #pragma unroll
@@ -251,7 +257,7 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
       const int q = m * FF::R0 + r;
       cx<W> v = {W(0), W(0)};
       if (FF::in_ok(t, m)) {
-        const cx<W> un = lds[cn * N + FF::in_pos(t, m, r)];
+        const cx<W> un = lds[rown * N + FF::in_pos(t, m, r)];
         v = {-(un.x * wo[0][q].x), -(un.y * wo[0][q].y)};
       }
       pr[0][q] = v;
@@ -262,7 +268,7 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   for (int m = 0; m < FI::ML; m++)
     if (FI::out_ok(t, m))
 #pragma unroll
-      for (int r = 0; r < FI::RL; r++) lds[c * N + FI::out_pos(t, m, r)] = wo[0][m * FI::RL + r];
+      for (int r = 0; r < FI::RL; r++) lds[row * N + FI::out_pos(t, m, r)] = wo[0][m * FI::RL + r];
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < FF::M0; m++)
@@ -270,21 +276,21 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
     for (int r = 0; r < FF::R0; r++) {
       const int q = m * FF::R0 + r;
       if (FF::in_ok(t, m)) {
-        const cx<W> wn = lds[cn * N + FF::in_pos(t, m, r)];
+        const cx<W> wn = lds[rown * N + FF::in_pos(t, m, r)];
         pr[0][q] = {uo[0][q].x * wn.x + pr[0][q].x, uo[0][q].y * wn.y + pr[0][q].y};
       }
     }
   __syncthreads();  // every w row has been read before the rows are reused
   // forward transform of P_{c+2} in row c, then its two rows unpacked
   cx<W> fout[1][FF::EL];
-  if constexpr (sizeof(W) == 4) FF::run(lay, lds, lds + 3 * N, t, pr, fout);
+  if constexpr (sizeof(W) == 4) FF::run(lay, lds, lds + 3 * G * N, t, pr, fout);
   else if constexpr (GT_X_PRETW) FF::run(lay, lds, tw, twr, t, pr, fout);
   else FF::run(lay, lds, tw, t, pr, fout);
   stage_sync<SY>();
   static_for<0, FF::ML * FF::RL>([&](auto qc) {
     constexpr int q = decltype(qc)::value, m = q / FF::RL, r = q % FF::RL;
     if constexpr (!(PZ && mid_slot<N, P, P * m + r * FF::NRL>())) {
-      if (FF::out_ok(t, m)) lds[c * N + FF::out_pos(t, m, r)] = fout[0][q];
+      if (FF::out_ok(t, m)) lds[row * N + FF::out_pos(t, m, r)] = fout[0][q];
     }
   });
   stage_sync<SY>();
@@ -293,7 +299,7 @@ __global__ __launch_bounds__(3 * P) void k_xfused3(KP kp, cx<W>* work, const cx<
   cx<W>* d1 = d0 + kp.nxp;
   for (int k = t; k <= kc; k += P) {
     const int kk = (k == 0) ? 0 : N - k;
-    const cx<W> Y = lds[c * N + k], Yc = lds[c * N + kk];
+    const cx<W> Y = lds[row * N + k], Yc = lds[row * N + kk];
     st_row_p<NT_C>(d0, (unsigned)k, cx<W>{(Y.x + Yc.x) * W(0.5), (Y.y - Yc.y) * W(0.5)});
     st_row_p<NT_C>(d1, (unsigned)k, cx<W>{(Y.y + Yc.y) * W(0.5), (Yc.x - Y.x) * W(0.5)});
   }
@@ -411,13 +417,15 @@ __global__ void k_forcing_seq(cx<double>* b, long long fs, const FEnt* __restric
 }
 
 // ---------------------------------------------------------------------------
-// the three-wave kernel applies: FP64, one wavefront per transform, and the
-// inverse's last-stage positions equal to the forward's first-stage ones
+// the three-wave kernel applies: FP64, whole transforms in each wavefront
+// (rows of 64 threads, or 16 or 32 with several row pairs per block), and
+// the inverse's last-stage positions equal to the forward's first-stage ones
 template <typename W, int N>
 constexpr bool use_x3() {
   using PL = Plan<N>;
-  return (sizeof(W) == 8 || GT_X3_FP32) && row_P(N) == 64 && PL::radix(0) == PL::radix(PL::S - 1) &&
-         cdiv(N / PL::radix(0), 64) == cdiv(N / PL::radix(PL::S - 1), 64);
+  constexpr int P = row_P(N);
+  return (sizeof(W) == 8 || GT_X3_FP32) && (P == 64 || P == 32 || P == 16) &&
+         PL::radix(0) == PL::radix(PL::S - 1) && cdiv(N / PL::radix(0), P) == cdiv(N / PL::radix(PL::S - 1), P);
 }
 
 template <typename W, int N>
@@ -425,12 +433,16 @@ static hipError_t xfused_n(const KP& kp, void* work, const void* tw, hipStream_t
   constexpr int P = row_P(N);
   dim3 grid(kp.ny / 2, kp.zn, kp.nmem);
   if constexpr (use_x3<W, N>()) {
-    const size_t lds3 = 3 * N * sizeof(cx<W>) + (sizeof(W) == 4 ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
-    if (pz_cut(N, kp.kxcut))
-      k_xfused3<W, N, P, true><<<grid, 3 * P, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
-    else
-      k_xfused3<W, N, P, false><<<grid, 3 * P, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
-    return hipGetLastError();
+    constexpr int G = 64 / P;  // row pairs per block
+    if ((kp.ny / 2) % G == 0) {
+      const size_t lds3 = 3 * G * N * sizeof(cx<W>) + (sizeof(W) == 4 ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
+      dim3 grid3(kp.ny / 2 / G, kp.zn, kp.nmem);
+      if (pz_cut(N, kp.kxcut))
+        k_xfused3<W, N, P, true><<<grid3, 3 * 64, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+      else
+        k_xfused3<W, N, P, false><<<grid3, 3 * 64, lds3, s>>>(kp, (cx<W>*)work, (const cx<W>*)tw);
+      return hipGetLastError();
+    }
   }
   const size_t lds6 = 6 * N * sizeof(cx<W>) + ((sizeof(W) == 4 && GT_TWLDS32 && GT_X_TWLDS32) ? Plan<N>::twsize() * sizeof(cx<W>) : 0);
   if (pz_cut(N, kp.kxcut))
