@@ -374,7 +374,7 @@ def main():
                          "instead of autotuning")
     ap.add_argument("--fusion", choices=["auto", "on", "off"], default="auto",
                     help="the z-forward passes fused with the z-inverse passes that follow them where compiled "
-                         "(auto: the library's default, on in FP64 and off in FP32-bulk)")
+                         "(auto: as the autotuner picks, or the library's default with --tiles)")
     ap.add_argument("--fd-directions", type=int, default=16, help="adjoint mode: central-difference checks")
     ap.add_argument("--legs", default=None,
                     help="step mode: comma-separated legs, the first giving the line's value (default "
@@ -454,8 +454,6 @@ def step_leg(args, precision, with_cpu):
 
     torch.cuda.init()
     ctx = lib.Context(p)
-    if args.fusion != "auto":
-        ctx.set_fusion(args.fusion == "on")
     ctx.set_state(b)
     tiles = None
     if args.tiles:
@@ -468,6 +466,8 @@ def step_leg(args, precision, with_cpu):
         # pencil widths of the z-inverse, z-forward and y passes and the z
         # chunk of the y/x/y passes (bit-neutral launch shapes)
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.autotune(3)))
+    if args.fusion != "auto":
+        ctx.set_fusion(args.fusion == "on")  # after the autotuner, which picks it otherwise
     if tiles is not None:
         tiles["zchunk"], tiles["zchunk_d"] = ctx.zchunk()
     seed = seed0

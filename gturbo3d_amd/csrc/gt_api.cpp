@@ -88,8 +88,8 @@ struct gt_ctx {
   // the pruned/dense mode (extents of every launch), the tile widths, the z chunk
   unsigned gen = 0;
   // the z-forward passes fused with the z-inverse passes that follow them
-  // where compiled (gt_set_fusion); on by default in FP64 (where it is
-  // measured faster), off in FP32-bulk
+  // where compiled (gt_set_fusion; gt_autotune times both); on by default
+  // in FP64, off in FP32-bulk
   bool fuse_ea = true;
   int tz = 0, ty = 0, tze = 0;       // pencils per block of the z-inverse / y / z-forward passes (0: default)
   double tune_ms = 0.0;              // the autotuner's time per nonlinear evaluation
@@ -127,8 +127,8 @@ struct gt_ctx {
   double* dtab = nullptr;    // fkx fky fkz fkxn fkyn fkzn
   int16_t* dkyext = nullptr;
   int16_t* dkzext = nullptr;
-  // the forcing entries of each column tile (FAux): CSR over ky index *
-  // (nxp / T) + kx tile, T the default z-forward width
+  // the forcing entries of each (ky, kx) column (FAux): CSR over ky index *
+  // nxp + kx index
   std::vector<int> fstart, fidx;
   int* dfstart = nullptr;
   int* dfidx = nullptr;
@@ -726,14 +726,14 @@ static int create_ctx(const gt_params* p, int nmem, const double* omegas, gt_ctx
                                                         : ((f.i == 1 && f.j == 1) || f.i == 1 || f.i == c->nxh);
       if (conj) offs.push_back(elem_off(c, f.i - 1, c->jc[f.j - 1] - 1, c->kc[f.k - 1] - 1));
     }
-    // entries by column tile, in table order within a tile (the order
+    // entries by (ky, kx) column, key = ky index * nxp + kx index (the
+    // offset within a z plane), in table order within a column (the order
     // forcing_entries emits them)
-    const int T = pencil_T(c->nz), ntx = c->nxp / T, zs = c->ny * c->nxp;
+    const long long zs = (long long)c->ny * c->nxp;
     std::vector<int> key(offs.size());
-    c->fstart.assign((size_t)c->ny * ntx + 1, 0);
+    c->fstart.assign((size_t)zs + 1, 0);
     for (size_t e = 0; e < offs.size(); e++) {
-      const long long r = offs[e] % zs;
-      key[e] = (int)(r / c->nxp) * ntx + (int)(r % c->nxp) / T;
+      key[e] = (int)(offs[e] % zs);
       c->fstart[key[e] + 1]++;
     }
     for (size_t k = 1; k < c->fstart.size(); k++) c->fstart[k] += c->fstart[k - 1];
@@ -1459,9 +1459,10 @@ int gt_autotune(gt_ctx* c, int nrep) {
   };
   c->tz = pick(tA, c->nz);
   c->tze = pick(tE, c->nz);
-  // the fused stage-1 z-forward pass (gt_set_fusion) is compiled at the
-  // default width only: with fusion on, the width that allows it is kept
-  if (c->fuse_ea) {
+  // the fused z-forward passes (gt_set_fusion) are compiled at the default
+  // width only for some lengths: with fusion on, a tuned width that they
+  // lack gives way to the default one
+  if (c->fuse_ea && !zfwd_zinv_supported(make_kp(c, false))) {
     const int te = c->tze;
     c->tze = 0;
     if (!zfwd_zinv_supported(make_kp(c, false))) c->tze = te;
@@ -1515,6 +1516,51 @@ int gt_autotune(gt_ctx* c, int nrep) {
     c->zchunk = std::get<1>(zcand[bi]);
     c->zchunk_d = std::get<2>(zcand[bi]);
     if (rc == GT_OK) c->tune_ms = tz[bi];  // the tuned time per evaluation
+    // fusion of the z passes, where the fused kernels exist: one step's
+    // passes with and without it, timed whole in two interleaved rounds
+    // (the best of each kept).  The state is only read: the new b goes to
+    // bh, and the forcing adds zeros.  Fused: passes B-D, E1 with stage 2's
+    // A, B-D, E2 with the next step's A; unfused: A, B-D, E1, A, B-D, E2.
+    if (rc == GT_OK && zfwd_zinv_supported(make_kp(c, false))) {
+      const KP kp = make_kp(c, false);
+      const size_t per = entries_per_step(c);
+      if (per > 0) {
+        rc = ensure_force_cap(c, per * c->nmem);
+        if (rc == GT_OK && hipMemsetAsync(c->dforce, 0, sizeof(FEnt) * per * c->nmem, c->stream) != hipSuccess)
+          rc = fail(GT_EHIP, "autotune forcing buffer");
+      }
+      FAux fx;
+      fx.ent = c->dforce;
+      fx.per = (int)per;
+      fx.start = c->dfstart;
+      fx.idx = c->dfidx;
+      auto unfused = [&]() -> int {
+        if (int r = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream, c->b)) return r;
+        return nonlinear_stage(c, kp, c->bh, EPI_STEP2, c->bh, c->stream, c->b);
+      };
+      auto fused = [&]() -> int {
+        if (int r = nonlinear_mid(c, kp, c->stream)) return r;
+        LAUNCH(PASS_EA, launch_zfwd_zinv(c->prec, 1, kp, c->work, c->b, nullptr, twp(c, 2), FAux{}, c->stream));
+        if (int r = nonlinear_mid(c, kp, c->stream)) return r;
+        LAUNCH(PASS_EF, launch_zfwd_zinv(c->prec, 2, kp, c->work, c->b, c->bh, twp(c, 2), fx, c->stream));
+        return GT_OK;
+      };
+      double tf[2] = {1e30, 1e30};  // unfused, fused
+      for (int round = 0; round < 2 && rc == GT_OK; round++)
+        for (int f = 0; f < 2 && rc == GT_OK; f++) {
+          if (f == 1) rc = launch_zinv(c->prec, 6, kp, c->b, c->work, twp(c, 2), c->stream) == hipSuccess
+                               ? GT_OK : fail(GT_EHIP, "autotune pass A");
+          if (rc == GT_OK) rc = f ? fused() : unfused();  // warm-up
+          if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune record");
+          for (int r = 0; r < nrep && rc == GT_OK; r++) rc = f ? fused() : unfused();
+          float t = 0.f;
+          if (rc == GT_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = fail(GT_EHIP, "autotune timing");
+          if (rc == GT_OK) tf[f] = std::min(tf[f], (double)t / nrep);
+        }
+      if (rc == GT_OK) c->fuse_ea = tf[1] < tf[0];
+    }
     if (e0) c->pool.push_back(e0);
     if (e1) c->pool.push_back(e1);
   }

@@ -68,9 +68,9 @@ struct FEnt {
 
 // One step's forcing for the z-forward pass that adds it (the stage-2 pass
 // fused with the next step's z-inverse pass): ent holds the step's entries,
-// per for each member in turn; the entries of column tile key = (ky index)
-// * (nxp / T) + (kx tile) are idx[start[key] .. start[key + 1]), member-
-// relative, in table order.
+// per for each member in turn; the entries of column key = (ky index) * nxp
+// + (kx index) are idx[start[key] .. start[key + 1]), member-relative, in
+// table order, so the columns of one kx tile are one contiguous range.
 struct FAux {
   const FEnt* ent = nullptr;
   int per = 0;

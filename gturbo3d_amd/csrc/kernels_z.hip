@@ -454,14 +454,14 @@ __attribute__((amdgpu_waves_per_eu(FA ? 1 : sizeof(W) == 8 ? GT_ZFWD_WPE64 : GT_
   constexpr bool FZR = FA == 1 ? GT_EA_FZREG : FA == 2 ? GT_EF_FZREG : 0;
   cx<double> bhr[FA ? 3 : 1][FA ? NS : 1];
   double fza[FZR ? NS : 1];
-  // FA = 2: this step's forcing entries of the block's column tile,
+  // FA = 2: this step's forcing entries of the block's T columns,
   // fx2.idx[fa0 .. fa1) (member-relative entry numbers, in table order)
   int fa0 = 0, fa1 = 0;
   const FEnt* fent = nullptr;
   if constexpr (FA == 2) {
-    const int key = j * (kp.nxp / T) + tx;
-    fa0 = fx2.start[key];
-    fa1 = fx2.start[key + 1];
+    const int key0 = j * kp.nxp + tx * T;
+    fa0 = fx2.start[key0];
+    fa1 = fx2.start[min(key0 + T, (j + 1) * kp.nxp)];
     fent = fx2.ent + (long long)mem * fx2.per;
   }
   if constexpr (FA) {
@@ -776,26 +776,30 @@ hipError_t launch_zfwd(int prec, int epi, const KP& kp, const void* work, const 
 // Pass E of stage 1 fused with pass A of stage 2, and pass E of stage 2
 // fused with the forcing and pass A of the next step (k_zfwd with FA 1 and
 // 2): they read the products and b and write the six z-inverse outputs into
-// the work fields (FA 2 also the new b).  Compiled for 512-point z transforms with simple viscosity at the
-// default width of 8 pencils (one block of 8 wavefronts per CU at its
-// register count; 4 pencils, two blocks per CU, measured slower: 1.19
-// against 1.03 ms per step in FP64, 1.63 against 0.78 in FP32-bulk).  The
-// z-forward width of the context must be the default one, which pass E of
-// stage 2 uses.
+// the work fields (FA 2 also the new b).  Compiled with simple viscosity for
+// 512-point z transforms at the default width of 8 pencils (one block of 8
+// wavefronts per CU at its register count; 4 pencils, two blocks per CU,
+// measured slower: 1.19 against 1.03 ms per step in FP64, 1.63 against 0.78
+// in FP32-bulk), where the z-forward width of the context must be the
+// default one, which pass E of stage 2 uses; and for 128-point transforms
+// at both widths.  gt_autotune decides whether a context uses them.
 bool zfwd_zinv_supported(const KP& kp) {
-  return kp.nz == 512 && kp.emx != nullptr && (alt_T(512) == 0 || kp.tze != alt_T(512));
+  if (kp.emx == nullptr) return false;
+  if (kp.nz == 128) return true;  // both widths
+  return kp.nz == 512 && (alt_T(512) == 0 || kp.tze != alt_T(512));
 }
 
-template <typename W>
+template <typename W, int N, int T>
 static hipError_t zfwd_zinv_w(int stage, const KP& kp, void* work, const void* b, void* out, const void* tw,
                               const FAux& fx, hipStream_t s) {
-  constexpr int N = 512, T = pencil_T(N), P = pencil_P(N);
+  constexpr int P = pencil_P(N);
   const int grid = (kp.kxcut + T) / T * kp.ny * kp.nmem;
   const cx<double>* bb = (const cx<double>*)b;
   cx<double>* o = (cx<double>*)out;
   cx<W>* wk = (cx<W>*)work;
   const cx<W>* t = (const cx<W>*)tw;
-  const bool pz = kp.pruned && pz_cut(N, kp.kzcut);
+  // the pruned slots as in the unfused z-forward pass (lengths of 256 and up)
+  const bool pz = pz_len(N) && kp.pruned && pz_cut(N, kp.kzcut);
   if (stage == 1) {
     // out is unused (bh stays in registers)
     if (pz) k_zfwd<W, N, T, P, EPI_STEP1, true, true, 1><<<grid, T * P, zlds<W, N, T>(), s>>>(kp, wk, bb, nullptr, t, wk, fx);
@@ -813,8 +817,15 @@ static hipError_t zfwd_zinv_w(int stage, const KP& kp, void* work, const void* b
 hipError_t launch_zfwd_zinv(int prec, int stage, const KP& kp, void* work, const void* b, void* out, const void* tw,
                             const FAux& fx, hipStream_t s) {
   if (!zfwd_zinv_supported(kp) || (stage != 1 && stage != 2)) return hipErrorInvalidValue;
-  return prec ? zfwd_zinv_w<float>(stage, kp, work, b, out, tw, fx, s)
-              : zfwd_zinv_w<double>(stage, kp, work, b, out, tw, fx, s);
+  if (kp.nz == 128) {
+    if (kp.tze == alt_T(128))
+      return prec ? zfwd_zinv_w<float, 128, alt_T(128)>(stage, kp, work, b, out, tw, fx, s)
+                  : zfwd_zinv_w<double, 128, alt_T(128)>(stage, kp, work, b, out, tw, fx, s);
+    return prec ? zfwd_zinv_w<float, 128, pencil_T(128)>(stage, kp, work, b, out, tw, fx, s)
+                : zfwd_zinv_w<double, 128, pencil_T(128)>(stage, kp, work, b, out, tw, fx, s);
+  }
+  return prec ? zfwd_zinv_w<float, 512, pencil_T(512)>(stage, kp, work, b, out, tw, fx, s)
+              : zfwd_zinv_w<double, 512, pencil_T(512)>(stage, kp, work, b, out, tw, fx, s);
 }
 
 }  // namespace gt

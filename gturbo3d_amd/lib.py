@@ -291,7 +291,8 @@ class Context:
     def set_fusion(self, on: bool) -> None:
         """Fuse the z-forward passes with the z-inverse passes that follow them
         (bh kept in registers; b not read back within one step call) where the
-        library compiles it.  On by default in FP64, off in FP32-bulk."""
+        library compiles it.  On by default in FP64, off in FP32-bulk; autotune()
+        keeps whichever runs a step faster."""
         _check(load().gt_set_fusion(self.h, 1 if on else 0), "gt_set_fusion")
 
     def fusion(self):

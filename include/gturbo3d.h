@@ -183,17 +183,18 @@ int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
 int gt_set_zchunk(gt_ctx* c, int planes, int with_d);
 int gt_get_zchunk(gt_ctx* c, int* planes, int* with_d);
 
-/* Cross-stage fusion (no counterpart in the reference; on by default in
- * FP64, off in FP32-bulk, where the fused kernels measured slower): the
- * z-forward pass of stage 1 keeps bh = emk (b + dt2 N1) in registers and
- * runs stage 2's z-inverse pass (curl bh, k^2 bh) on it in the same kernel,
- * so bh never goes through device memory; within one gt_step call, the
- * z-forward pass of stage 2 likewise adds the step's forcing, stores b and
- * runs the next step's z-inverse pass on it, so b is not read back.
- * Results equal the unfused step to rounding (contraction order).  Compiled for NZ = 512 with simple
- * viscosity (alpnu = 1, mu = 0) at the default z-forward width, which
- * gt_autotune keeps while fusion is on; *active (may be NULL) reports
- * whether the current context uses it. */
+/* Cross-stage fusion (no counterpart in the reference): the z-forward pass
+ * of stage 1 keeps bh = emk (b + dt2 N1) in registers and runs stage 2's
+ * z-inverse pass (curl bh, k^2 bh) on it in the same kernel, so bh never
+ * goes through device memory; within one gt_step call, the z-forward pass
+ * of stage 2 likewise adds the step's forcing, stores b and runs the next
+ * step's z-inverse pass on it, so b is not read back.  Results equal the
+ * unfused step to rounding (contraction order).  Compiled with simple
+ * viscosity (alpnu = 1, mu = 0) for NZ = 512 at the default z-forward width
+ * (which gt_autotune keeps while fusion is on) and for NZ = 128 at both.
+ * On by default in FP64, off in FP32-bulk; gt_autotune keeps whichever runs
+ * a step faster, and gt_set_fusion after it forces a choice.  *active (may
+ * be NULL) reports whether the current context uses it. */
 int gt_set_fusion(gt_ctx* c, int on);
 int gt_get_fusion(gt_ctx* c, int* on, int* active);
 

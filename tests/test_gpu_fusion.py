@@ -212,3 +212,48 @@ def test_fused_ensemble_graphs_512(require_gpu):
         assert err < 1e-14
     ens.close()
     A.close()
+
+
+@pytest.mark.parametrize("tze", [8, 16])
+def test_fused_steps_128(require_gpu, tze):
+    """The fused kernels of 128-point z transforms, at both z-forward widths:
+    three fused steps against three unfused ones and against the oracle."""
+    p = Params(nx=128, ny=128, nz=128, nu=1e-3, dt=4e-3, omega=2.0, k1f=2.0, k2f=3.0)
+    p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
+    say(f"fused steps 128^3, {tze} pencils")
+    b = initial.initial_state(p, seed=13)
+    bf, sf, pruned = run(p, b, 7, 3, True, tiles=(0, tze, 0))
+    assert pruned
+    bu, su, _ = run(p, b, 7, 3, False, tiles=(0, tze, 0))
+    assert sf == su
+    err = rel_l2(bf, bu)
+    say(f"  fused vs unfused, 3 steps: {err:.2e}")
+    assert err < 1e-14
+    ob, os_ = ol.Oracle(p).step(b, 7, 3)
+    assert sf == os_
+    err = rel_l2(bf, ob)
+    say(f"  fused vs oracle, 3 steps: {err:.2e}")
+    assert err < 1e-12
+
+
+def test_autotune_keeps_a_fusion_choice(require_gpu):
+    """gt_autotune times the step with and without fusion where the fused
+    kernels exist and keeps one; the result does not depend on it."""
+    p = Params(nx=128, ny=128, nz=128, nu=1e-3, dt=4e-3, omega=2.0, famp=0.05, k1f=2.0, k2f=3.0)
+    b = initial.initial_state(p, seed=14)
+    out = []
+    with lib.Context(p) as g:
+        g.set_state(b)
+        g.autotune(2)
+        on, active = g.fusion()
+        assert active == on
+        say(f"  autotune at 128^3: fusion {'on' if on else 'off'}, tiles {g.tiles()}")
+        g.set_state(b)
+        s1 = g.step(2, 9)
+        out.append(g.get_state())
+        g.set_fusion(not on)
+        g.set_state(b)
+        s2 = g.step(2, 9)
+        out.append(g.get_state())
+    assert s1 == s2
+    assert rel_l2(out[0], out[1]) < 1e-14
