@@ -155,9 +155,11 @@ struct gt_ctx {
     cx<double>* seg = nullptr;      // the states of one segment, every states
     cx<double>* work = nullptr;     // 9 fields
     cx<double>* vec = nullptr;      // 7 x 3 fields: lam (two, alternating) nu g bh h g1
-    FEnt* dunit = nullptr;          // unit-amplitude forcing entries of one segment
-    double* ddots = nullptr;
+    FEnt* dunit = nullptr;          // unit-amplitude forcing entries of every step
+    double* ddots = nullptr;        // their dots with the cotangents
     size_t cap_ckpt = 0, cap_seg = 0, cap_unit = 0;
+    hipStream_t rstream = nullptr;  // the segment recompute, beside the sweep
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // recompute done x2, sweep done x2
   } adj;
 
   // profiling
@@ -630,6 +632,9 @@ void free_ctx(gt_ctx* c) {
   hipFree(c->adj.vec);
   hipFree(c->adj.dunit);
   hipFree(c->adj.ddots);
+  for (auto e : c->adj.ev)
+    if (e) hipEventDestroy(e);
+  if (c->adj.rstream) hipStreamDestroy(c->adj.rstream);
   hipFree(c->ddiag_part);
   hipFree(c->ddiag_out);
   if (c->hforce) hipHostFree(c->hforce);
@@ -956,9 +961,10 @@ static void fill_forcing(const gt_ctx* c, int nsteps, int32_t* seeds, FEnt* buf)
 
 // Compute the forcing of nsteps steps on the host and queue its upload to
 // dforce on the context stream (advancing seeds).
-static int stage_forcing(gt_ctx* c, int nsteps, int32_t* seeds) {
+static int stage_forcing(gt_ctx* c, int nsteps, int32_t* seeds, hipStream_t st = nullptr) {
   const size_t per = entries_per_step(c);
   const size_t need = per * (size_t)c->nmem * (size_t)nsteps;
+  if (!st) st = c->stream;
   if (need > 0) {
     if (int rc = ensure_force_cap(c, need)) return rc;
     if (c->hforce_pending) {
@@ -966,8 +972,8 @@ static int stage_forcing(gt_ctx* c, int nsteps, int32_t* seeds) {
       c->hforce_pending = false;
     }
     fill_forcing(c, nsteps, seeds, c->hforce);
-    HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * need, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->hforce_done, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dforce, c->hforce, sizeof(FEnt) * need, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(c->hforce_done, st));
     c->hforce_pending = true;
   } else {
     fill_forcing(c, nsteps, seeds, nullptr);
@@ -1846,6 +1852,12 @@ int adj_reserve(gt_ctx* c, size_t nckpt, size_t nseg_states, size_t nunit) {
     if (int rc = adj_malloc((void**)&A.work, sizeof(cx<double>) * 9 * c->fs, "work fields")) return rc;
     if (int rc = adj_malloc((void**)&A.vec, 7 * st, "cotangent fields")) return rc;
   }
+  if (!A.rstream) {
+    // default priority, like the context stream
+    if (hipStreamCreate(&A.rstream) != hipSuccess) return fail(GT_EHIP, "adjoint: recompute stream");
+    for (auto& e : A.ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(GT_EHIP, "adjoint: events");
+  }
   if (nckpt > A.cap_ckpt) {
     hipFree(A.ckpt);
     A.cap_ckpt = 0;
@@ -1986,55 +1998,96 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   if (!lx || !ly || !lz || !gx || !gy || !gz) return fail(GT_EINVAL, "gt_adjoint_backward: null field");
   if (grad_amps && nshells != (int)c->shell_k2.size())
     return fail(GT_EINVAL, "gt_adjoint_backward: expected " + std::to_string(c->shell_k2.size()) + " shells");
-  hipStream_t st = c->stream;
   const KP kp = make_kp1(c, false, 0);
   const long long f3 = 3 * c->fs;
   const size_t sbytes = sizeof(cx<double>) * f3;
   const size_t per = entries_per_step(c);
-  const int K = A.every;
+  const int K = A.every, nseg = A.nseg, nsteps = A.nsteps;
+  // two buffers of segment states (the recompute of one segment fills one
+  // while the sweep of the next reads the other), and the unit increments
+  // and forcing dots of every step
+  if (int rc = adj_reserve(c, (size_t)nseg + 1, 2 * (size_t)K, std::max<size_t>(per * nsteps, 1))) return rc;
+  if (per > 0)
+    if (int rc = ensure_force_cap(c, per * K)) return rc;
+  hipStream_t st = c->stream;
+  // In the reference stage-1 mode the sweep of segment s (context stream)
+  // and the recompute of segment s - 1 (a second stream) touch disjoint
+  // buffers, so they run side by side.  The midpoint sweep evaluates the
+  // stage-1 term through the context's work fields, which the recompute
+  // uses too: there both run on the context stream.
+  const bool overlap = c->dt2 == 0.0;
+  hipStream_t rs = overlap ? A.rstream : st;
+  const bool prof = c->prof;
+  c->prof = false;  // its events go to the context stream
+  struct ProfRestore {
+    gt_ctx* c;
+    bool on;
+    ~ProfRestore() { c->prof = on; }
+  } prof_restore{c, prof};
   HIPCHK(hipMemsetAsync(A.work, 0, sizeof(cx<double>) * 9 * c->fs, st));
   HIPCHK(hipMemsetAsync(A.vec, 0, 7 * sbytes, st));
   // the recomputed states are stored by pass E, which in pruned mode writes
   // only the kept modes: the rest of every slot must be zero
-  if (K > 1) HIPCHK(hipMemsetAsync(A.seg + f3, 0, (size_t)(K - 1) * sbytes, st));
-  cx<double>* lam = A.vec;          // the cotangent of the current state
+  HIPCHK(hipMemsetAsync(A.seg, 0, 2 * (size_t)K * sbytes, st));
+  // unit-amplitude increments of every step, from each segment's seed
+  std::vector<FEnt> hunit(std::max<size_t>(per * nsteps, 1));
+  for (int s = 0; s < nseg; s++) {
+    int32_t sd = A.seeds[s];
+    const int len = std::min(K, nsteps - s * K);
+    for (int i = 0; i < len; i++)
+      forcing_entries(c, &sd, per ? hunit.data() + per * ((size_t)s * K + i) : nullptr, 0, true);
+  }
+  if (per) HIPCHK(hipMemcpyAsync(A.dunit, hunit.data(), sizeof(FEnt) * per * nsteps, hipMemcpyHostToDevice, st));
+  cx<double>* lam = A.vec;  // the cotangent of the current state
   cx<double>* lam_next = A.vec + f3;
   const double* src[3] = {lx, ly, lz};
   if (int rc = fields_to_device(c, src, lam, A.work, st)) return rc;
   if (kp.pruned) HIPCHK(launch_adj_project(kp, lam, st));
-  std::vector<double> gsh(c->shell_k2.size(), 0.0);
-  std::vector<FEnt> hunit(std::max<size_t>(per * K, 1));
-  std::vector<double> hdots(std::max<size_t>(per * K, 1));
-  for (int s = A.nseg - 1; s >= 0; s--) {
-    const int len = std::min(K, A.nsteps - s * K);
-    // unit-amplitude increments of the segment's steps
-    int32_t sd = A.seeds[s];
-    for (int i = 0; i < len; i++) forcing_entries(c, &sd, per ? hunit.data() + per * i : nullptr, 0, true);
-    if (per) HIPCHK(hipMemcpyAsync(A.dunit, hunit.data(), sizeof(FEnt) * per * len, hipMemcpyHostToDevice, st));
-    // recompute the segment's states b_{sK} .. b_{sK+len-1}: the first is
-    // the checkpoint itself, each next one is stepped out of place into its
-    // own slot (slot 0 of A.seg is unused)
+  if (overlap) {
+    HIPCHK(hipEventRecord(A.ev[2], st));  // the cleared slots
+    HIPCHK(hipStreamWaitEvent(rs, A.ev[2], 0));
+  }
+  // state i of segment s: the checkpoint, or slot i of the segment's buffer
+  auto slot = [&](int s, int i) -> cx<double>* {
+    return i == 0 ? A.ckpt + (size_t)s * f3 : A.seg + ((size_t)(s & 1) * K + i) * f3;
+  };
+  auto recompute = [&](int s) -> int {
+    const int len = std::min(K, nsteps - s * K);
     int32_t sr = A.seeds[s];
     if (len > 1)
-      if (int rc = stage_forcing(c, len - 1, &sr)) return rc;
-    auto state = [&](int i) -> cx<double>* { return i == 0 ? A.ckpt + (size_t)s * f3 : A.seg + (size_t)i * f3; };
+      if (int rc = stage_forcing(c, len - 1, &sr, rs)) return rc;
     for (int i = 0; i + 1 < len; i++)
-      if (int rc = enqueue_step_to(c, st, state(i), state(i + 1), c->dforce + per * i)) return rc;
+      if (int rc = enqueue_step_to(c, rs, slot(s, i), slot(s, i + 1), c->dforce + per * i)) return rc;
+    HIPCHK(hipEventRecord(A.ev[s & 1], rs));
+    return GT_OK;
+  };
+  if (int rc = recompute(nseg - 1)) return rc;
+  for (int s = nseg - 1; s >= 0; s--) {
+    const int len = std::min(K, nsteps - s * K);
+    if (overlap) HIPCHK(hipStreamWaitEvent(st, A.ev[s & 1], 0));
+    if (s > 0) {
+      // segment s - 1 refills the buffer that the sweep of segment s + 1 read
+      if (overlap && s + 1 < nseg) HIPCHK(hipStreamWaitEvent(rs, A.ev[2 + ((s + 1) & 1)], 0));
+      if (int rc = recompute(s - 1)) return rc;
+    }
     for (int i = len - 1; i >= 0; i--) {
-      if (per) HIPCHK(launch_adj_fdot(lam, c->fs, A.dunit + per * i, (int)per, A.ddots + per * i, st));
-      if (int rc = step_adjoint(c, kp, state(i), lam, lam_next)) return rc;
+      const size_t n = (size_t)s * K + i;
+      if (per) HIPCHK(launch_adj_fdot(lam, c->fs, A.dunit + per * n, (int)per, A.ddots + per * n, st));
+      if (int rc = step_adjoint(c, kp, slot(s, i), lam, lam_next)) return rc;
       std::swap(lam, lam_next);
     }
-    if (per) HIPCHK(hipMemcpyAsync(hdots.data(), A.ddots, sizeof(double) * per * len, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (int i = 0; i < len; i++)
-      for (size_t e = 0; e < per; e++) gsh[c->entry_shell[e]] += hdots[per * i + e];
+    if (overlap) HIPCHK(hipEventRecord(A.ev[2 + (s & 1)], st));
   }
+  std::vector<double> hdots(std::max<size_t>(per * nsteps, 1));
+  if (per) HIPCHK(hipMemcpyAsync(hdots.data(), A.ddots, sizeof(double) * per * nsteps, hipMemcpyDeviceToHost, st));
   double* dst[3] = {gx, gy, gz};
   if (int rc = fields_to_host(c, lam, dst, A.work, st)) return rc;
   // leave the context at the forward run's final state
-  HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)A.nseg * f3, sbytes, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)nseg * f3, sbytes, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  std::vector<double> gsh(c->shell_k2.size(), 0.0);
+  for (int n = 0; n < nsteps; n++)
+    for (size_t e = 0; e < per; e++) gsh[c->entry_shell[e]] += hdots[per * n + e];
   if (grad_amps)
     for (size_t q = 0; q < gsh.size(); q++) grad_amps[q] = gsh[q];
   return GT_OK;
