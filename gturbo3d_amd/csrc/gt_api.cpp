@@ -605,6 +605,7 @@ void build_shells(gt_ctx* c) {
 void free_ctx(gt_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->adj.rstream) hipStreamSynchronize(c->adj.rstream);
   for (auto& r : c->recs) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -1848,6 +1849,7 @@ int adj_reserve(gt_ctx* c, size_t nckpt, size_t nseg_states, size_t nunit) {
   gt_ctx::Adjoint& A = c->adj;
   const size_t st = sizeof(cx<double>) * 3 * c->fs;
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (A.rstream) HIPCHK(hipStreamSynchronize(A.rstream));  // nothing of an earlier backward in flight
   if (!A.work) {
     if (int rc = adj_malloc((void**)&A.work, sizeof(cx<double>) * 9 * c->fs, "work fields")) return rc;
     if (int rc = adj_malloc((void**)&A.vec, 7 * st, "cotangent fields")) return rc;
