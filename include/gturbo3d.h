@@ -166,8 +166,10 @@ int gt_prof_read(gt_ctx* c, double* ms, int64_t* launches, double* bytes, int np
  * evaluations of the nonlinear term of the current state for each pencil
  * width (8 or 16 pencils per block) of the z-inverse and z-forward passes
  * and keeps the fastest of each, then the width of the y passes together
- * with the z chunk below, by whole-evaluation time.  Results do not depend
- * on the widths.
+ * with the z chunk below, by whole-evaluation time, and last, where the
+ * fused z passes exist (gt_set_fusion), whether to use them, by the time of
+ * a step's passes.  Results do not depend on the widths or the chunk, and
+ * on fusion only to rounding.
  * gt_set_tiles forces a triple (0 = default); gt_get_tiles reports it and
  * the tuned time per evaluation in ms (0 before tuning). */
 int gt_autotune(gt_ctx* c, int nrep);
