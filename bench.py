@@ -370,8 +370,8 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pencil widths")
-    ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk[,with_d]]: force the pencil widths (and the z chunk of passes B-D) "
-                         "instead of autotuning")
+    ap.add_argument("--tiles", default="", help="tz,tze,ty[,zchunk[,with_d[,zstreams]]]: force the pencil widths (and the "
+                         "z chunk of passes B-D and its stream count) instead of autotuning")
     ap.add_argument("--fusion", choices=["auto", "on", "off"], default="auto",
                     help="the z-forward passes fused with the z-inverse passes that follow them where compiled "
                          "(auto: as the autotuner picks, or the library's default with --tiles)")
@@ -461,6 +461,8 @@ def step_leg(args, precision, with_cpu):
         ctx.set_tiles(*tv[:3])
         if len(tv) > 3:
             ctx.set_zchunk(tv[3], len(tv) < 5 or bool(tv[4]))
+        if len(tv) > 5:
+            ctx.set_zstreams(tv[5])
         tiles = dict(zip(("tz", "tze", "ty", "ms_per_nonlinear"), ctx.tiles()))
     elif not args.no_autotune:
         # pencil widths of the z-inverse, z-forward and y passes and the z
@@ -470,6 +472,7 @@ def step_leg(args, precision, with_cpu):
         ctx.set_fusion(args.fusion == "on")  # after the autotuner, which picks it otherwise
     if tiles is not None:
         tiles["zchunk"], tiles["zchunk_d"] = ctx.zchunk()
+        tiles["zstreams"] = ctx.zstreams()
     seed = seed0
     seed = ctx.step(args.warmup, seed)
     ctx.sync()

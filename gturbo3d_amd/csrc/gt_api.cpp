@@ -87,6 +87,11 @@ struct gt_ctx {
   // bumped whenever something an ensemble's captured graphs bake in changes:
   // the pruned/dense mode (extents of every launch), the tile widths, the z chunk
   unsigned gen = 0;
+  // z chunks of passes B-D on one stream or alternating between two
+  // (gt_set_zstreams); the second stream and its fork/join events
+  int zstreams = 1;
+  hipStream_t cstream = nullptr;
+  hipEvent_t cev[2] = {nullptr, nullptr};
   // the z-forward passes fused with the z-inverse passes that follow them
   // where compiled (gt_set_fusion; gt_autotune times both); on by default
   // in FP64, off in FP32-bulk
@@ -426,22 +431,31 @@ struct ProfScope {
   gt_ctx* c;
   int pass;
   hipEvent_t a = nullptr;
-  ProfScope(gt_ctx* c_, int pass_) : c(c_), pass(pass_) {
+  hipStream_t s;
+  // events on the stream the pass is launched on (the context stream by default)
+  ProfScope(gt_ctx* c_, int pass_, hipStream_t s_ = nullptr) : c(c_), pass(pass_), s(s_ ? s_ : c_->stream) {
     if (c->prof) {
       a = pool_get(c);
-      if (a) hipEventRecord(a, c->stream);
+      if (a) hipEventRecord(a, s);
     }
   }
   ~ProfScope() {
     if (c->prof && a) {
       hipEvent_t b = pool_get(c);
       if (b) {
-        hipEventRecord(b, c->stream);
+        hipEventRecord(b, s);
         c->recs.push_back({pass, a, b});
       }
     }
   }
 };
+
+#define LAUNCH_ON(pass, stream, expr)                                                            \
+  do {                                                                                          \
+    ProfScope ps_(c, pass, stream);                                                             \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail(GT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
 
 #define LAUNCH(pass, expr)                                                                      \
   do {                                                                                          \
@@ -459,13 +473,32 @@ int nonlinear_mid(gt_ctx* c, const KP& kp, hipStream_t st) {
   // picked by gt_autotune (0: all planes at once); results do not depend on it.
   const int zc = (c->zchunk > 0 && c->zchunk < c->nz) ? c->zchunk : c->nz;
   const bool dch = c->zchunk_d || zc == c->nz;
-  for (int z0 = 0; z0 < c->nz; z0 += zc) {
+  // zstreams = 2: odd chunks on a second stream, so that two chunks overlap
+  // (gt_set_zstreams; the autotuner picks it with the chunk size).  Per-pass
+  // timing records each launch's events on its own stream.
+  const bool two = c->zstreams == 2 && zc < c->nz && dch;
+  if (two && !c->cstream) {
+    // default priority, like the context stream
+    HIPCHK(hipStreamCreate(&c->cstream));
+    for (auto& e : c->cev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (two) {
+    HIPCHK(hipEventRecord(c->cev[0], st));
+    HIPCHK(hipStreamWaitEvent(c->cstream, c->cev[0], 0));
+  }
+  int k = 0;
+  for (int z0 = 0; z0 < c->nz; z0 += zc, k++) {
     KP kc = kp;
     kc.z0 = z0;
     kc.zn = std::min(zc, c->nz - z0);
-    LAUNCH(PASS_B, launch_yfft(pr, +1, 6, kc, c->work, twp(c, 1), st));
-    LAUNCH(PASS_C, launch_xfused(pr, kc, c->work, twp(c, 0), st));
-    if (dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), st));
+    hipStream_t cs = (two && (k & 1)) ? c->cstream : st;
+    LAUNCH_ON(PASS_B, cs, launch_yfft(pr, +1, 6, kc, c->work, twp(c, 1), cs));
+    LAUNCH_ON(PASS_C, cs, launch_xfused(pr, kc, c->work, twp(c, 0), cs));
+    if (dch) LAUNCH_ON(PASS_D, cs, launch_yfft(pr, -1, 3, kc, c->work, twp(c, 1), cs));
+  }
+  if (two) {
+    HIPCHK(hipEventRecord(c->cev[1], c->cstream));
+    HIPCHK(hipStreamWaitEvent(st, c->cev[1], 0));
   }
   if (!dch) LAUNCH(PASS_D, launch_yfft(pr, -1, 3, kp, c->work, twp(c, 1), st));
   return GT_OK;
@@ -606,6 +639,12 @@ void free_ctx(gt_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->adj.rstream) hipStreamSynchronize(c->adj.rstream);
+  if (c->cstream) {
+    hipStreamSynchronize(c->cstream);
+    hipStreamDestroy(c->cstream);
+  }
+  for (auto e : c->cev)
+    if (e) hipEventDestroy(e);
   for (auto& r : c->recs) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -1376,6 +1415,21 @@ int gt_set_zchunk(gt_ctx* c, int planes, int with_d) {
   return GT_OK;
 }
 
+int gt_set_zstreams(gt_ctx* c, int nstreams) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nstreams != 1 && nstreams != 2) return fail(GT_EINVAL, "gt_set_zstreams: 1 or 2 streams");
+  if (c->zstreams != nstreams) c->gen++;
+  c->zstreams = nstreams;
+  return GT_OK;
+}
+
+int gt_get_zstreams(gt_ctx* c, int* nstreams) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nstreams) return fail(GT_EINVAL, "gt_get_zstreams: null");
+  *nstreams = c->zstreams;
+  return GT_OK;
+}
+
 int gt_set_fusion(gt_ctx* c, int on) {
   if (int rc = check_ctx(c)) return rc;
   if (c->fuse_ea != (on != 0)) c->gen++;
@@ -1480,11 +1534,11 @@ int gt_autotune(gt_ctx* c, int nrep) {
   // picked widths that were slower in the chunked step, and the chunks add
   // launches that per-pass sums would not count
   if (rc == GT_OK) {
-    // (y width, planes, pass D in the chunks)
+    // (y width, planes, pass D in the chunks, streams)
     std::vector<int> ycand;
     for (int w : cand)
       if (ok(c->ny, w)) ycand.push_back(enc(c->ny, w));
-    std::vector<std::tuple<int, int, bool>> zcand;
+    std::vector<std::tuple<int, int, bool, int>> zcand;
     // planes per chunk: NZ/16 .. NZ/2, and 3, 5 and 6 NZ/64 (a ragged last
     // chunk; at 512^3 FP64, 40 planes measured 0.7% faster than 32)
     std::vector<int> planes;
@@ -1492,10 +1546,13 @@ int gt_autotune(gt_ctx* c, int nrep) {
       if (c->nz % d == 0 && c->nz / d >= 8) planes.push_back(c->nz / d);
     for (int m : {3, 5, 6})
       if ((m * c->nz) % 64 == 0 && m * c->nz / 64 >= 8) planes.push_back(m * c->nz / 64);
+    // (one stream: the two-stream chunks of gt_set_zstreams overlap each
+    // pass with the other stream's, which per-pass timing cannot separate;
+    // the autotuner keeps the configuration whose passes it can account for)
     for (int yw : ycand) {
-      zcand.push_back({yw, 0, true});
+      zcand.push_back({yw, 0, true, 1});
       for (int pl : planes)
-        for (bool wd : {true, false}) zcand.push_back({yw, pl, wd});
+        for (bool wd : {true, false}) zcand.push_back({yw, pl, wd, 1});
     }
     hipEvent_t e0 = pool_get(c), e1 = pool_get(c);
     if (!e0 || !e1) rc = fail(GT_EHIP, "autotune events");
@@ -1506,6 +1563,7 @@ int gt_autotune(gt_ctx* c, int nrep) {
         c->ty = std::get<0>(zcand[i]);
         c->zchunk = std::get<1>(zcand[i]);
         c->zchunk_d = std::get<2>(zcand[i]);
+        c->zstreams = std::get<3>(zcand[i]);
         const KP kp = make_kp(c, false);
         rc = nonlinear_stage(c, kp, c->b, EPI_STEP1, c->bh, c->stream);  // warm-up
         if (rc == GT_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(GT_EHIP, "autotune record");
@@ -1522,6 +1580,7 @@ int gt_autotune(gt_ctx* c, int nrep) {
     c->ty = std::get<0>(zcand[bi]);
     c->zchunk = std::get<1>(zcand[bi]);
     c->zchunk_d = std::get<2>(zcand[bi]);
+    c->zstreams = std::get<3>(zcand[bi]);
     if (rc == GT_OK) c->tune_ms = tz[bi];  // the tuned time per evaluation
     // fusion of the z passes, where the fused kernels exist: one step's
     // passes with and without it, timed whole in two interleaved rounds

@@ -28,7 +28,7 @@ EXPORTS = [
     "gt_batch_create", "gt_batch_size", "gt_batch_set_state", "gt_batch_get_state", "gt_batch_step",
     "gt_batch_spectra", "gt_ensemble_create", "gt_ensemble_destroy", "gt_ensemble_set_seeds",
     "gt_ensemble_get_seeds", "gt_ensemble_step", "gt_ensemble_sync", "gt_autotune", "gt_set_tiles",
-    "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_set_fusion", "gt_get_fusion", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
+    "gt_get_tiles", "gt_set_zchunk", "gt_get_zchunk", "gt_set_zstreams", "gt_get_zstreams", "gt_set_fusion", "gt_get_fusion", "gt_forcing_shells", "gt_set_shell_amps", "gt_adjoint_forward", "gt_adjoint_backward",
     "gt_check_finite", "gt_spectra_noalloc",
 ]
 
@@ -94,6 +94,8 @@ def load():
     L.gt_autotune.argtypes = [vp, ip]
     L.gt_set_tiles.argtypes = [vp, ip, ip, ip]
     L.gt_set_zchunk.argtypes = [vp, ip, ip]
+    L.gt_set_zstreams.argtypes = [vp, ip]
+    L.gt_get_zstreams.argtypes = [vp, ctypes.POINTER(ip)]
     L.gt_set_fusion.argtypes = [vp, ip]
     L.gt_get_fusion.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     L.gt_get_zchunk.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
@@ -287,6 +289,15 @@ class Context:
         """z planes per chunk of the y-inverse and x passes (0: all at once); with_d:
         the y-forward pass runs in the chunks too (else once after them)."""
         _check(load().gt_set_zchunk(self.h, int(planes), 1 if with_d else 0), "gt_set_zchunk")
+
+    def set_zstreams(self, n: int) -> None:
+        """The z chunks of passes B-D on one stream or alternating between two."""
+        _check(load().gt_set_zstreams(self.h, int(n)), "gt_set_zstreams")
+
+    def zstreams(self) -> int:
+        v = ctypes.c_int()
+        _check(load().gt_get_zstreams(self.h, ctypes.byref(v)), "gt_get_zstreams")
+        return v.value
 
     def set_fusion(self, on: bool) -> None:
         """Fuse the z-forward passes with the z-inverse passes that follow them

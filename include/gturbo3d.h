@@ -184,6 +184,17 @@ int gt_get_tiles(gt_ctx* c, int* tz, int* tze, int* ty, double* ms);
  * with_d may be NULL in gt_get_zchunk.  No counterpart in the reference. */
 int gt_set_zchunk(gt_ctx* c, int planes, int with_d);
 int gt_get_zchunk(gt_ctx* c, int* planes, int* with_d);
+/* The z chunks on one stream (1, the default) or alternating between the
+ * context stream and a second one (2), so that two chunks overlap; it
+ * applies when the chunks are fewer planes than NZ and include the
+ * y-forward pass.  At 512^3 FP64, two streams with chunks of 16 or 20
+ * planes measured 2% faster steps than one with 40, but a pass's launches
+ * then share the device with the other stream's, so per-pass timing
+ * (gt_prof_*, events on each launch's stream) no longer separates the
+ * passes; gt_autotune keeps one stream.  Results do not depend on it.
+ * No counterpart in the reference. */
+int gt_set_zstreams(gt_ctx* c, int nstreams);
+int gt_get_zstreams(gt_ctx* c, int* nstreams);
 
 /* Cross-stage fusion (no counterpart in the reference): the z-forward pass
  * of stage 1 keeps bh = emk (b + dt2 N1) in registers and runs stage 2's

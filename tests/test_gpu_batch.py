@@ -72,6 +72,7 @@ def test_ensemble_graph_replays_match_eager(require_gpu):
     # passes B-D of batch A in z chunks (a ragged last one), pass D of B whole:
     # the captured graphs hold every chunk launch
     A.set_zchunk(12)
+    A.set_zstreams(2)  # the chunks of A alternate between two streams in the graphs
     B.set_zchunk(8, False)
     for m in range(3):
         A.set_member(m, ics_a[m])
@@ -118,19 +119,23 @@ def test_tile_widths_do_not_change_results(require_gpu, n):
                     results[(tz, tze, ty)] = g.get_state()
         # z chunks of passes B-D: equal chunks, a ragged last chunk, one plane
         g.set_tiles(tz0, tze0, ty0)
-        for zc, wd in ((n // 4, True), (3 * n // 8 + 1, True), (1, True), (n // 4, False)):
+        # and the chunks alternating between two streams
+        for zc, wd, ns in ((n // 4, True, 1), (3 * n // 8 + 1, True, 1), (1, True, 1), (n // 4, False, 1),
+                           (n // 4, True, 2), (3 * n // 8 + 1, True, 2), (1, True, 2)):
             g.set_zchunk(zc, wd)
-            assert g.zchunk() == (zc, wd)
+            g.set_zstreams(ns)
+            assert g.zchunk() == (zc, wd) and g.zstreams() == ns
             g.set_state(b)
             g.step(2, 3)
-            results[("zchunk", zc, wd)] = g.get_state()
+            results[("zchunk", zc, wd, ns)] = g.get_state()
         g.set_zchunk(0)
+        g.set_zstreams(1)
         tz, tze, ty, ms = g.autotune(2)
         assert tz in (8, 16) and tze in (8, 16) and ty in (8, 16) and ms > 0
-        # the autotuner's candidates: NZ/16 .. NZ/2 and 3, 5, 6 NZ/64 planes
+        # the autotuner's candidates: NZ/16 .. NZ/2 and 3, 5, 6 NZ/64 planes, one stream
         cands = {0} | {n // d for d in (16, 8, 4, 2) if n % d == 0 and n // d >= 8}
         cands |= {m * n // 64 for m in (3, 5, 6) if (m * n) % 64 == 0 and m * n // 64 >= 8}
-        assert g.zchunk()[0] in cands
+        assert g.zchunk()[0] in cands and g.zstreams() == 1
     ref = results[(tz0, tze0, ty0)]
     for key, st in results.items():
         for x, y in zip(st, ref):
