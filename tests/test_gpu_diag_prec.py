@@ -144,3 +144,47 @@ def test_fp32_bulk_512_workload(require_gpu):
     assert max(e_fields) < 1e-5
     assert e_sp < 1e-5
     assert e_hel < 1e-4
+
+
+def test_fp32_bulk_512_500_steps(require_gpu):
+    """The precision workload at its full length: 500 steps at 512^3 (nu =
+    5e-5, Omega = 2, 3D3C forcing (2,3] at rate 0.1, dt = 1e-3) in FP32-bulk
+    and in FP64 from the same state and seed, stage 1 in the midpoint mode
+    (with dt2 = 0 this workload diverges within 500 steps in both
+    precisions).  Error per field and per diagnostic: fields 1e-5, energy
+    spectrum 1e-5 (shells above 1e-12 of the peak), helicity spectrum 1e-4,
+    energy 1e-6, dissipation 1e-5."""
+    import math as _m
+    from gturbo3d_amd.params import DT2_MIDPOINT
+    base = dict(nu=5e-5, omega=2.0, dt=1e-3, k1f=2.0, k2f=3.0, famp=0.0, dt2_mode=DT2_MIDPOINT)
+    p64 = cfg(512, **base)
+    p64.famp = _m.sqrt(0.1 / lib.forcing_count(p64)[0])
+    p32 = cfg(512, **{**base, "famp": p64.famp, "precision": PREC_FP32_BULK})
+    say("fp32-bulk 512^3: 500 midpoint steps against FP64")
+    b = initial.initial_state(p64, seed=0)
+    out = {}
+    for name, p in (("fp32", p32), ("fp64", p64)):
+        with lib.Context(p) as g:
+            g.set_state(b)
+            s = g.step(500, 0)
+            st = g.get_state()
+            sp, _, hel, glob = g.spectra()
+        out[name] = (s, st, sp, hel, glob)
+    (s32, b32, sp32, hel32, gl32), (s64, b64, sp64, hel64, gl64) = out["fp32"], out["fp64"]
+    assert s32 == s64
+    assert all(np.isfinite(f).all() for f in b64)
+    e_fields = [rel_l2([x], [y]) for x, y in zip(b32, b64)]
+    m = sp64 > 1e-12 * sp64.max()
+    e_sp = float(np.max(np.abs(sp32[m] - sp64[m]) / sp64[m]))
+    mh = np.abs(hel64) > 1e-6 * np.abs(hel64).max()
+    e_hel = float(np.max(np.abs(hel32[mh] - hel64[mh]) / np.abs(hel64[mh])))
+    # energy and dissipation: the sums of the first three and the next three glob terms
+    e_E = abs(gl32[:3].sum() - gl64[:3].sum()) / gl64[:3].sum()
+    e_D = abs(gl32[3:6].sum() - gl64[3:6].sum()) / gl64[3:6].sum()
+    say(f"  fields {[f'{e:.2e}' for e in e_fields]}; energy spectrum {e_sp:.2e}; helicity spectrum {e_hel:.2e}; "
+        f"energy {e_E:.2e}; dissipation {e_D:.2e}")
+    assert max(e_fields) < 1e-5
+    assert e_sp < 1e-5
+    assert e_hel < 1e-4
+    assert e_E < 1e-6
+    assert e_D < 1e-5
