@@ -80,9 +80,9 @@ __host__ __device__ constexpr PlanInfo plan_of(int n) {
     case 48: return {3, {3, 2, 8, 0, 0, 0}};
     case 64: return {2, {8, 8, 0, 0, 0, 0}};
     case 96: return {3, {3, 4, 8, 0, 0, 0}};
-    case 128: return {3, {8, 2, 8, 0, 0, 0}};  // first radix = last: the three-wave x pass applies
+    case 128: return {3, {8, 2, 8, 0, 0, 0}};  // first radix = last (three-wave x pass, fused z passes)
     case 192: return {3, {3, 8, 8, 0, 0, 0}};
-    case 256: return {3, {4, 8, 8, 0, 0, 0}};
+    case 256: return {3, {8, 4, 8, 0, 0, 0}};  // first radix = last (three-wave x pass, fused z passes)
     case 384: return {4, {3, 2, 8, 8, 0, 0}};
     case 512: return {3, {8, 8, 8, 0, 0, 0}};
     case 768: return {4, {3, 4, 8, 8, 0, 0}};

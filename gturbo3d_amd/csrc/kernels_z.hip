@@ -781,11 +781,12 @@ hipError_t launch_zfwd(int prec, int epi, const KP& kp, const void* work, const 
 // wavefronts per CU at its register count; 4 pencils, two blocks per CU,
 // measured slower: 1.19 against 1.03 ms per step in FP64, 1.63 against 0.78
 // in FP32-bulk), where the z-forward width of the context must be the
-// default one, which pass E of stage 2 uses; and for 128-point transforms
-// at both widths.  gt_autotune decides whether a context uses them.
+// default one, which pass E of stage 2 uses; and for 128- and 256-point
+// transforms at both widths.  gt_autotune decides whether a context uses
+// them.
 bool zfwd_zinv_supported(const KP& kp) {
   if (kp.emx == nullptr) return false;
-  if (kp.nz == 128) return true;  // both widths
+  if (kp.nz == 128 || kp.nz == 256) return true;  // both widths
   return kp.nz == 512 && (alt_T(512) == 0 || kp.tze != alt_T(512));
 }
 
@@ -823,6 +824,13 @@ hipError_t launch_zfwd_zinv(int prec, int stage, const KP& kp, void* work, const
                   : zfwd_zinv_w<double, 128, alt_T(128)>(stage, kp, work, b, out, tw, fx, s);
     return prec ? zfwd_zinv_w<float, 128, pencil_T(128)>(stage, kp, work, b, out, tw, fx, s)
                 : zfwd_zinv_w<double, 128, pencil_T(128)>(stage, kp, work, b, out, tw, fx, s);
+  }
+  if (kp.nz == 256) {
+    if (kp.tze == alt_T(256))
+      return prec ? zfwd_zinv_w<float, 256, alt_T(256)>(stage, kp, work, b, out, tw, fx, s)
+                  : zfwd_zinv_w<double, 256, alt_T(256)>(stage, kp, work, b, out, tw, fx, s);
+    return prec ? zfwd_zinv_w<float, 256, pencil_T(256)>(stage, kp, work, b, out, tw, fx, s)
+                : zfwd_zinv_w<double, 256, pencil_T(256)>(stage, kp, work, b, out, tw, fx, s);
   }
   return prec ? zfwd_zinv_w<float, 512, pencil_T(512)>(stage, kp, work, b, out, tw, fx, s)
               : zfwd_zinv_w<double, 512, pencil_T(512)>(stage, kp, work, b, out, tw, fx, s);

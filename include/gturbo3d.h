@@ -204,7 +204,8 @@ int gt_get_zstreams(gt_ctx* c, int* nstreams);
  * step's z-inverse pass on it, so b is not read back.  Results equal the
  * unfused step to rounding (contraction order).  Compiled with simple
  * viscosity (alpnu = 1, mu = 0) for NZ = 512 at the default z-forward width
- * (which gt_autotune keeps while fusion is on) and for NZ = 128 at both.
+ * (which gt_autotune keeps while fusion is on) and for NZ = 128 and 256 at
+ * both.
  * On by default in FP64, off in FP32-bulk; gt_autotune keeps whichever runs
  * a step faster, and gt_set_fusion after it forces a choice.  *active (may
  * be NULL) reports whether the current context uses it. */

@@ -214,13 +214,14 @@ def test_fused_ensemble_graphs_512(require_gpu):
     A.close()
 
 
-@pytest.mark.parametrize("tze", [8, 16])
-def test_fused_steps_128(require_gpu, tze):
-    """The fused kernels of 128-point z transforms, at both z-forward widths:
-    three fused steps against three unfused ones and against the oracle."""
-    p = Params(nx=128, ny=128, nz=128, nu=1e-3, dt=4e-3, omega=2.0, k1f=2.0, k2f=3.0)
+@pytest.mark.parametrize("n,tze", [(128, 8), (128, 16), (256, 8), (256, 16)])
+def test_fused_steps_128_256(require_gpu, n, tze):
+    """The fused kernels of 128- and 256-point z transforms, at both z-forward
+    widths: three fused steps against three unfused ones and against the
+    oracle."""
+    p = Params(nx=n, ny=n, nz=n, nu=1e-3, dt=4e-3, omega=2.0, k1f=2.0, k2f=3.0)
     p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
-    say(f"fused steps 128^3, {tze} pencils")
+    say(f"fused steps {n}^3, {tze} pencils")
     b = initial.initial_state(p, seed=13)
     bf, sf, pruned = run(p, b, 7, 3, True, tiles=(0, tze, 0))
     assert pruned

@@ -36,7 +36,7 @@ def cfg(n, **kw):
     return Params(**base)
 
 
-@pytest.mark.parametrize("n", [16, 24, 32, 48, 64, 96, 128, (32, 16, 64), (64, 32, 16), (128, 24, 32)])
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 64, 96, 128, 256, (32, 16, 64), (64, 32, 16), (128, 24, 32), (256, 32, 16)])
 def test_nonlinear_matches_oracle(require_gpu, n):
     p = cfg(n, omega=2.0)
     say(f"nonlinear {p.nx}x{p.ny}x{p.nz}")
