@@ -9,15 +9,15 @@ nonlinear and Coriolis terms advance by forward Euler; this configuration
 then diverges between steps 300 and 400 (E = 0.5694 at step 300, 6.69e82 at
 step 400, identically in the oracle and on the GPU).
 
-    python tests/golden/make_cfg2_spectra.py [--steps 2000] [--threads 6]
+    python tests/golden/make_rotating256_spectra.py [--steps 2000] [--threads 6]
 
-Writes tests/golden/cfg2_spectra_256.npz.  Takes about 1.8 h on 6 cores.
+Writes tests/golden/rotating256_midpoint_spectra.npz.  Takes about 1.8 h on 6 cores.
 
 With --mode reference the same run uses the reference's stage 1 (dt2 = 0)
 for 300 steps, records every 100, and writes
-tests/golden/cfg2_reference_256.npz (about 16 minutes on 8 cores):
+tests/golden/rotating256_reference_spectra.npz (about 16 minutes on 8 cores):
 
-    python tests/golden/make_cfg2_spectra.py --mode reference
+    python tests/golden/make_rotating256_spectra.py --mode reference
 """
 import argparse
 import math
@@ -37,7 +37,7 @@ from gturbo3d_amd.params import FORCING_2D2C, Params  # noqa: E402
 import oracle_lib as ol  # noqa: E402
 
 
-def cfg2_params(n=256, dt2_mode=1):
+def rotating256_params(n=256, dt2_mode=1):
     # famp = sqrt(0.1 / nf) with nf = 10 forced modes on (3, 4]; dt2_mode 1:
     # midpoint stage 1, 0: the reference's (dt2 = 0)
     return Params(nx=n, ny=n, nz=n, nu=2e-4, alpnu=1.0, mu=0.0, alpmu=0.0, dt=2e-3, omega=5.0,
@@ -58,9 +58,9 @@ def main():
     if a.steps is None:
         a.steps = 300 if ref else 2000
     if a.out is None:
-        a.out = os.path.join(HERE, "cfg2_reference_256.npz" if ref else "cfg2_spectra_256.npz")
+        a.out = os.path.join(HERE, "rotating256_reference_spectra.npz" if ref else "rotating256_midpoint_spectra.npz")
     ol.set_threads(a.threads)
-    p = cfg2_params(a.n, dt2_mode=0 if ref else 1)
+    p = rotating256_params(a.n, dt2_mode=0 if ref else 1)
     o = ol.Oracle(p)
     assert o.nf == 10, o.nf
     b = initial.initial_state(p, seed=0)

@@ -206,7 +206,7 @@ def test_ensemble_follows_batch_changes_after_capture(require_gpu):
     A.close()
 
 
-def test_ensemble_cfg4_shapes(require_gpu):
+def test_ensemble_128_and_96_workload_shapes(require_gpu):
     """The batched-ensemble workload at its shapes: 32 members at 128^3 and 16
     at 96^3 (FP64, nu = 1e-3, dt = 4e-3, 3D3C (2,3] at rate 0.1, seeds 0..47,
     Omega ~ U[0.5, 10] from PCG64(48)) in one ensemble of captured graphs.

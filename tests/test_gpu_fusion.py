@@ -33,7 +33,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def cfg512(**kw):
+def params512(**kw):
     base = dict(nx=512, ny=512, nz=512, nu=5e-5, dt=1e-3, omega=2.0, k1f=2.0, k2f=3.0)
     base.update(kw)
     p = Params(**base)
@@ -56,7 +56,7 @@ def run(p, b, seed, nsteps, fused, tiles=None):
 
 @pytest.mark.parametrize("dt2", [DT2_REFERENCE, DT2_MIDPOINT])
 def test_fused_step_512_fp64(require_gpu, dt2):
-    p = cfg512(dt2_mode=dt2)
+    p = params512(dt2_mode=dt2)
     say(f"fused step 512^3 FP64, dt2 mode {dt2}")
     b = initial.initial_state(p, seed=5)
     bf, sf, pruned = run(p, b, 8299396, 2, True)
@@ -79,7 +79,7 @@ def test_fused_step_512_fp64(require_gpu, dt2):
 def test_fused_steps_one_call_equal_separate_calls(require_gpu):
     """Three steps in one call (two stage-2 passes fused with the next step's
     z-inverse pass and the forcing) against three one-step calls (none)."""
-    p = cfg512()
+    p = params512()
     say("fused 3 steps in one call vs 3 calls")
     b = initial.initial_state(p, seed=9)
     b3, s3, _ = run(p, b, 4242, 3, True)
@@ -97,7 +97,7 @@ def test_fused_steps_one_call_equal_separate_calls(require_gpu):
 
 def test_fused_step_512_2d2c(require_gpu):
     """2D2C forcing (bz only, conjugate entries on the kx = 0 plane)."""
-    p = cfg512(variant=FORCING_2D2C, k1f=3.0, k2f=4.0, famp=0.1, omega=5.0)
+    p = params512(variant=FORCING_2D2C, k1f=3.0, k2f=4.0, famp=0.1, omega=5.0)
     say("fused steps 512^3 2D2C")
     b = initial.initial_state(p, seed=10)
     bf, sf, _ = run(p, b, 77, 3, True)
@@ -109,12 +109,12 @@ def test_fused_step_512_2d2c(require_gpu):
 
 
 def test_fused_step_512_fp32_bulk(require_gpu):
-    p = cfg512(precision=PREC_FP32_BULK)
+    p = params512(precision=PREC_FP32_BULK)
     say("fused step 512^3 FP32-bulk")
     with lib.Context(p) as g:
         # off by default in FP32-bulk, where the fused kernels measured slower
         assert g.fusion() == (False, False)
-    b = initial.initial_state(cfg512(), seed=6)
+    b = initial.initial_state(params512(), seed=6)
     bf, sf, _ = run(p, b, 17, 2, True)
     bu, su, _ = run(p, b, 17, 2, False)
     assert sf == su
@@ -125,10 +125,10 @@ def test_fused_step_512_fp32_bulk(require_gpu):
 
 def test_fused_step_512_dense(require_gpu):
     """An untruncated state runs the dense path: full extents, no pruned slots."""
-    p = cfg512(truncate=0)
+    p = params512(truncate=0)
     say("fused step 512^3 dense")
     rng = np.random.default_rng(8)
-    b = initial.initial_state(cfg512(), seed=8)
+    b = initial.initial_state(params512(), seed=8)
     b[0] = b[0] + 1e-7 * (rng.standard_normal(b[0].shape) + 1j * rng.standard_normal(b[0].shape))
     b[0][0, 0, 0] = 0.0
     bf, sf, pruned = run(p, b, 3, 1, True)
@@ -144,7 +144,7 @@ def test_fusion_inactive_at_other_widths_and_sizes(require_gpu):
     """The fused kernel exists for NZ = 512 at the default z-forward width;
     elsewhere the setting is kept and the two passes run (the same bits as
     with fusion off)."""
-    p = cfg512()
+    p = params512()
     with lib.Context(p) as g:
         g.set_tiles(0, 16, 0)
         assert g.fusion() == (True, False)
@@ -165,7 +165,7 @@ def test_fused_batch_members_equal_solo_runs(require_gpu):
     """Two members at 512^3 in one batch, two steps (the member index in the
     fused kernels' work, state and forcing offsets) against solo fused runs,
     bit for bit."""
-    p = cfg512()
+    p = params512()
     say("fused batch 2 x 512^3")
     b0 = initial.initial_state(p, seed=11)
     b1 = initial.initial_state(p, seed=12)
@@ -176,7 +176,7 @@ def test_fused_batch_members_equal_solo_runs(require_gpu):
         seeds = g.step_batch(2, [21, 22])
         m0, m1 = g.get_member(0), g.get_member(1)
     s0, ss0, _ = run(p, b0, 21, 2, True)
-    q = cfg512(omega=3.5)
+    q = params512(omega=3.5)
     q.famp = p.famp
     s1, ss1, _ = run(q, b1, 22, 2, True)
     assert list(seeds) == [ss0, ss1]
@@ -189,7 +189,7 @@ def test_fused_ensemble_graphs_512(require_gpu):
     graph: the graphs hold the fused kernels, their forcing entries come from
     the ensemble's own buffers.  Five steps (two replays and an eager
     remainder) against solo fused runs of five steps in one call."""
-    p = cfg512()
+    p = params512()
     say("fused ensemble 2 x 512^3, chunk 2")
     oms, seeds = [2.0, 3.5], [31, 32]
     ics = [initial.initial_state(p, seed=40 + m) for m in range(2)]
@@ -203,7 +203,7 @@ def test_fused_ensemble_graphs_512(require_gpu):
     ens.sync()
     got_seeds = ens.get_seeds(0)
     for m in range(2):
-        q = cfg512(omega=oms[m])
+        q = params512(omega=oms[m])
         q.famp = p.famp
         ref, rs, _ = run(q, ics[m], seeds[m], 5, True)
         assert got_seeds[m] == rs

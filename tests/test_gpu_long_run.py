@@ -1,7 +1,7 @@
 """Long-run parity: the 256^3 rotating run (nu = 2e-4, Omega = 5, 2D2C forcing
 on (3, 4], dt = 2e-3, 2000 steps, midpoint stage 1) against the CPU oracle's
-spectra recorded every 100 steps (tests/golden/cfg2_spectra_256.npz, written
-by tests/golden/make_cfg2_spectra.py).
+spectra recorded every 100 steps (tests/golden/rotating256_midpoint_spectra.npz, written
+by tests/golden/make_rotating256_spectra.py).
 
 Bar: time-averaged energy and kz = 0 spectra within 1% on every shell that
 holds energy, energy and dissipation (from glob) within 1% at every record,
@@ -9,7 +9,7 @@ and the forcing seed after 2000 steps equal.
 
 The same run in the reference's stage-1 mode (dt2 = 0) diverges between
 steps 300 and 400 in both codes; over its first 300 steps (records every
-100, tests/golden/cfg2_reference_256.npz) the spectra and glob of every
+100, tests/golden/rotating256_reference_spectra.npz) the spectra and glob of every
 record must match the oracle's to 1e-6.
 """
 import os
@@ -22,12 +22,12 @@ from gturbo3d_amd import initial, lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from make_cfg2_spectra import cfg2_params  # noqa: E402
+from make_rotating256_spectra import rotating256_params  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-FIXTURE = os.path.join(HERE, "golden", "cfg2_spectra_256.npz")
-FIXTURE_REF = os.path.join(HERE, "golden", "cfg2_reference_256.npz")
+FIXTURE = os.path.join(HERE, "golden", "rotating256_midpoint_spectra.npz")
+FIXTURE_REF = os.path.join(HERE, "golden", "rotating256_reference_spectra.npz")
 
 
 def say(msg):
@@ -39,7 +39,7 @@ def say(msg):
 def test_long_run_spectra_match_oracle(require_gpu):
     fx = np.load(FIXTURE)
     steps = fx["steps"]
-    p = cfg2_params(int(fx["n"]))
+    p = rotating256_params(int(fx["n"]))
     say(f"long run {p.nx}^3, {int(steps[-1])} steps, records every {int(steps[1] - steps[0])}")
     b = initial.initial_state(p, seed=0)
     sps, sp2ds, globs = [], [], []
@@ -72,7 +72,7 @@ def test_long_run_spectra_match_oracle(require_gpu):
 def test_reference_mode_run_matches_oracle(require_gpu):
     fx = np.load(FIXTURE_REF)
     steps = fx["steps"]
-    p = cfg2_params(int(fx["n"]), dt2_mode=int(fx["dt2_mode"]))
+    p = rotating256_params(int(fx["n"]), dt2_mode=int(fx["dt2_mode"]))
     assert p.dt2_mode == 0
     say(f"reference-mode run {p.nx}^3, {int(steps[-1])} steps, records every {int(steps[1] - steps[0])}")
     b = initial.initial_state(p, seed=0)
