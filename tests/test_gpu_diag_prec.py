@@ -188,3 +188,64 @@ def test_fp32_bulk_512_500_steps(require_gpu):
     assert e_hel < 1e-4
     assert e_E < 1e-6
     assert e_D < 1e-5
+
+
+def test_fp32_bulk_1024_100_steps(require_gpu):
+    """The precision workload at 1024^3 (nu = 5e-5, Omega = 2, 3D3C forcing
+    (2,3] at rate 0.1, dt = 1e-3, midpoint stage 1): 100 steps in FP32-bulk
+    and in FP64 from the same state and seed.  With dt = 1e-3 the resolved
+    high-k modes of this grid grow by about 1.17 per step (kmax u_max dt ~ 1.1),
+    so FP32 rounding noise there reaches O(1) before step 500; 100 steps is
+    the run both precisions keep bounded.  Bars: fields 1e-6, energy and
+    helicity spectra 1e-4 (shells above 1e-12 and 1e-6 of the peak), energy
+    spectrum 1e-5 on the shells above 1e-6 of the peak, energy 1e-6,
+    dissipation 1e-5.  Fields are compared plane by plane to bound the host
+    memory (three 8.6 GB fields per state)."""
+    import math as _m
+    from gturbo3d_amd.params import DT2_MIDPOINT
+    base = dict(nu=5e-5, omega=2.0, dt=1e-3, k1f=2.0, k2f=3.0, famp=0.0, dt2_mode=DT2_MIDPOINT)
+    p64 = cfg(1024, **base)
+    p64.famp = _m.sqrt(0.1 / lib.forcing_count(p64)[0])
+    p32 = cfg(1024, **{**base, "famp": p64.famp, "precision": PREC_FP32_BULK})
+    say("fp32-bulk 1024^3: 100 midpoint steps against FP64")
+    b = initial.initial_state(p64, seed=0)
+    out = {}
+    for name, p in (("fp64", p64), ("fp32", p32)):
+        with lib.Context(p) as g:
+            g.set_state(b)
+            if name == "fp32":
+                del b
+            s = g.step(100, 0)
+            sp, _, hel, glob = g.spectra()
+            st = g.get_state()
+        out[name] = (s, st, sp, hel, glob)
+        say(f"  {name}: 100 steps done, E = {sp.sum():.6f}")
+    (s32, b32, sp32, hel32, gl32), (s64, b64, sp64, hel64, gl64) = out["fp32"], out["fp64"]
+    del out
+    assert s32 == s64
+    e_fields = []
+    for x, y in zip(b32, b64):
+        num = den = 0.0
+        for k in range(0, y.shape[0], 64):
+            yk = y[k:k + 64]
+            assert np.isfinite(yk).all()
+            num += float(np.sum(np.abs(x[k:k + 64] - yk) ** 2))
+            den += float(np.sum(np.abs(yk) ** 2))
+        e_fields.append(math.sqrt(num / den))
+    del b32, b64
+    m = sp64 > 1e-12 * sp64.max()
+    e_sp = float(np.max(np.abs(sp32[m] - sp64[m]) / sp64[m]))
+    m6 = sp64 > 1e-6 * sp64.max()
+    e_sp6 = float(np.max(np.abs(sp32[m6] - sp64[m6]) / sp64[m6]))
+    mh = np.abs(hel64) > 1e-6 * np.abs(hel64).max()
+    e_hel = float(np.max(np.abs(hel32[mh] - hel64[mh]) / np.abs(hel64[mh])))
+    e_E = abs(gl32[:3].sum() - gl64[:3].sum()) / gl64[:3].sum()
+    e_D = abs(gl32[3:6].sum() - gl64[3:6].sum()) / gl64[3:6].sum()
+    say(f"  fields {[f'{e:.2e}' for e in e_fields]}; energy spectrum {e_sp:.2e} (shells above 1e-6 of the peak "
+        f"{e_sp6:.2e}); helicity spectrum {e_hel:.2e}; energy {e_E:.2e}; dissipation {e_D:.2e}")
+    assert max(e_fields) < 1e-6
+    assert e_sp < 1e-4
+    assert e_sp6 < 1e-5
+    assert e_hel < 1e-4
+    assert e_E < 1e-6
+    assert e_D < 1e-5
