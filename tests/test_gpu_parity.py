@@ -74,6 +74,39 @@ def test_ten_steps_match_oracle(require_gpu, n, variant, dt2):
     assert rel_l2(b10, ob10) < 1e-12
 
 
+def test_plain_64_workload_100_steps(require_gpu):
+    """The plain 64^3 workload at its full length (nu = 1e-3, Omega = 1,
+    3D3C forcing (2,3] at injection rate 0.1 from seed 0, dt = 5e-3, the
+    reference's stage 1, 100 steps) against the oracle: fields 1e-12 after
+    10 and after 100 steps (measured 2.0e-15 and 1.9e-14), energy spectrum
+    and glob 1e-12, the seed equal."""
+    p = cfg(64, famp=0.0)
+    p.famp = math.sqrt(0.1 / lib.forcing_count(p)[0])
+    say("plain 64^3 workload, 100 steps")
+    b = initial.initial_state(p, seed=0)
+    with lib.Context(p) as g:
+        g.set_state(b)
+        s10 = g.step(10, 0)
+        b10 = g.get_state()
+        s100 = g.step(90, s10)
+        b100 = g.get_state()
+        sp, _, _, glob = g.spectra()
+    o = ol.Oracle(p)
+    ob10, os10 = o.step(b, 0, 10)
+    ob100, os100 = o.step(ob10, os10, 90)
+    assert s10 == os10 and s100 == os100
+    e10, e100 = rel_l2(b10, ob10), rel_l2(b100, ob100)
+    osp, _ = o.spectrum(ob100)
+    m = osp > 1e-12 * osp.max()
+    e_sp = float(np.max(np.abs(sp[m] - osp[m]) / osp[m]))
+    og = o.glob(ob100)
+    e_g = float(np.max(np.abs(glob - og) / np.maximum(np.abs(og), 1e-12 * np.abs(og).max())))
+    say(f"  fields after 10 steps {e10:.2e}, after 100 {e100:.2e}; energy spectrum {e_sp:.2e}; glob {e_g:.2e}")
+    assert e10 < 1e-12
+    assert e100 < 1e-12
+    assert e_sp < 1e-12 and e_g < 1e-12
+
+
 @pytest.mark.parametrize("n", [128, 256])
 def test_one_step_large_matches_oracle(require_gpu, n):
     p = cfg(n, nu=2e-4, dt=2e-3, omega=5.0)
