@@ -32,6 +32,14 @@ def to_complex(t: torch.Tensor):
     return [np.ascontiguousarray(c[f]) for f in range(3)]
 
 
+def _host_empty(shape, dtype=torch.float64) -> torch.Tensor:
+    """An uninitialised CPU tensor from torch's caching host allocator when a
+    GPU is present: pinned, so the library's copies to and from it go by DMA,
+    and reused from call to call, so that filling it does not page-fault
+    through fresh memory (51 MB per state at 128^3)."""
+    return torch.empty(shape, dtype=dtype, pin_memory=torch.cuda.is_available())
+
+
 def _fields_of(t: torch.Tensor):
     """The three complex fields viewing the memory of a CPU float64 tensor
     (3, NZ, NY, NXH, 2): the library reads or writes them in place."""
@@ -53,7 +61,7 @@ class StepRun(torch.autograd.Function):
         ctx.seed_out = g.adjoint_forward(int(nsteps), int(seed), int(every))
         ctx.g = g
         ctx.run = g.adjoint_run
-        out = torch.empty((3,) + tuple(g.shape) + (2,), dtype=torch.float64)
+        out = _host_empty((3,) + tuple(g.shape) + (2,))
         g.get_state(out=_fields_of(out))
         return out.to(b0.device)
 
@@ -64,7 +72,7 @@ class StepRun(torch.autograd.Function):
                                f"not to this node's run {ctx.run} (one live run per context)")
         # the library writes dJ/db_0 straight into the result tensor's memory
         # (its real view is the three complex fields back to back)
-        res = torch.empty(grad.shape, dtype=torch.float64)
+        res = _host_empty(grad.shape)
         _, ga = ctx.g.adjoint_backward(to_complex(grad), out=_fields_of(res))
         dev = grad.device
         return res.to(dev), torch.from_numpy(ga.copy()).to(dev), None, None, None, None
@@ -92,6 +100,27 @@ def band_weights(p: Params, kmin: float, kmax: float) -> np.ndarray:
     return w[..., None]
 
 
+class _BandEnergy(torch.autograd.Function):
+    """J = sum w b^2 and its gradient 2 w b, each one pass over b into a
+    buffer of the caching host allocator (_host_empty) for CPU tensors."""
+
+    @staticmethod
+    def forward(ctx, b, w):
+        ctx.save_for_backward(b, w)
+        s = _host_empty(b.shape, b.dtype) if b.device.type == "cpu" else torch.empty_like(b)
+        torch.mul(b, b, out=s)
+        s.mul_(w)
+        return s.sum()
+
+    @staticmethod
+    def backward(ctx, gout):
+        b, w = ctx.saved_tensors
+        g = _host_empty(b.shape, b.dtype) if b.device.type == "cpu" else torch.empty_like(b)
+        torch.mul(b, w, out=g)
+        g.mul_(2.0 * gout)
+        return g, None
+
+
 def band_energy(b: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     """J = sum_k w |b|^2 over the real view b (3, NZ, NY, NXH, 2)."""
-    return (w * b * b).sum()
+    return _BandEnergy.apply(b, w.to(b.device, b.dtype))
