@@ -155,6 +155,9 @@ struct gt_ctx {
   struct Adjoint {
     int nsteps = 0, every = 0, nseg = 0;
     bool ready = false;
+    // the forward left the states of its last segment in the segment buffer
+    // (its half (nseg - 1) & 1), so the backward need not recompute them
+    bool last_kept = false;
     std::vector<int32_t> seeds;     // forcing seed at the start of each segment
     cx<double>* ckpt = nullptr;     // nseg + 1 states (the last one: the final state)
     cx<double>* seg = nullptr;      // the states of one segment, every states
@@ -2034,20 +2037,42 @@ int gt_adjoint_forward(gt_ctx* c, int nsteps, int32_t* seed, int every) {
   const int K = std::min(every, nsteps);
   const int nseg = (nsteps + K - 1) / K;
   const size_t per = entries_per_step(c);
-  if (int rc = adj_reserve(c, (size_t)nseg + 1, (size_t)K, std::max<size_t>(per * K, 1))) return rc;
-  const size_t st = sizeof(cx<double>) * 3 * c->fs;
+  // the segment buffer at the backward's size (two halves of K states)
+  if (int rc = adj_reserve(c, (size_t)nseg + 1, 2 * (size_t)K, std::max<size_t>(per * K, 1))) return rc;
+  A.last_kept = false;
+  const long long f3 = 3 * c->fs;
+  const size_t st = sizeof(cx<double>) * f3;
   A.seeds.assign(nseg, 0);
   for (int s = 0; s < nseg; s++) {
-    HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)s * 3 * c->fs, c->b, st, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)s * f3, c->b, st, hipMemcpyDeviceToDevice, c->stream));
     A.seeds[s] = *seed;
-    if (int rc = step_m(c, std::min(K, nsteps - s * K), seed)) return rc;
+    const int len = std::min(K, nsteps - s * K);
+    if (s + 1 < nseg) {
+      if (int rc = step_m(c, len, seed)) return rc;
+      continue;
+    }
+    // The last segment, the first the backward sweeps, runs out of place
+    // through its slots of the segment buffer (state i of the segment in
+    // slot i, as the backward's recompute leaves them), so the backward
+    // starts at once instead of recomputing it first.  Pass E in pruned
+    // mode writes only the kept modes: the slots are cleared first.
+    if (c->ext_stream) HIPCHK(hipStreamSynchronize(c->ext_stream));
+    cx<double>* half = A.seg + (size_t)(s & 1) * K * f3;
+    if (len > 1) HIPCHK(hipMemsetAsync(half + f3, 0, (size_t)(len - 1) * st, c->stream));
+    if (int rc = stage_forcing(c, len, seed)) return rc;
+    for (int i = 0; i < len; i++) {
+      const cx<double>* in = i == 0 ? A.ckpt + (size_t)s * f3 : half + (size_t)i * f3;
+      cx<double>* out = i + 1 < len ? half + (size_t)(i + 1) * f3 : c->b;
+      if (int rc = enqueue_step_to(c, c->stream, in, out, c->dforce + per * i)) return rc;
+    }
   }
-  HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)nseg * 3 * c->fs, c->b, st, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(A.ckpt + (size_t)nseg * f3, c->b, st, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   A.nsteps = nsteps;
   A.every = K;
   A.nseg = nseg;
   A.ready = true;
+  A.last_kept = true;
   return GT_OK;
 }
 
@@ -2067,7 +2092,13 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   // two buffers of segment states (the recompute of one segment fills one
   // while the sweep of the next reads the other), and the unit increments
   // and forcing dots of every step
+  // the half of the segment buffer holding the forward's last segment (-1:
+  // none); a call that fails leaves it unset, so the next one recomputes
+  const cx<double>* seg0 = A.seg;
+  int kept = A.last_kept ? ((nseg - 1) & 1) : -1;
+  A.last_kept = false;
   if (int rc = adj_reserve(c, (size_t)nseg + 1, 2 * (size_t)K, std::max<size_t>(per * nsteps, 1))) return rc;
+  if (A.seg != seg0) kept = -1;
   if (per > 0)
     if (int rc = ensure_force_cap(c, per * K)) return rc;
   hipStream_t st = c->stream;
@@ -2089,7 +2120,8 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   HIPCHK(hipMemsetAsync(A.vec, 0, 7 * sbytes, st));
   // the recomputed states are stored by pass E, which in pruned mode writes
   // only the kept modes: the rest of every slot must be zero
-  HIPCHK(hipMemsetAsync(A.seg, 0, 2 * (size_t)K * sbytes, st));
+  for (int h = 0; h < 2; h++)
+    if (h != kept) HIPCHK(hipMemsetAsync(A.seg + (size_t)h * K * f3, 0, (size_t)K * sbytes, st));
   // unit-amplitude increments of every step, from each segment's seed
   std::vector<FEnt> hunit(std::max<size_t>(per * nsteps, 1));
   for (int s = 0; s < nseg; s++) {
@@ -2122,7 +2154,11 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
     HIPCHK(hipEventRecord(A.ev[s & 1], rs));
     return GT_OK;
   };
-  if (int rc = recompute(nseg - 1)) return rc;
+  if (kept < 0) {
+    if (int rc = recompute(nseg - 1)) return rc;
+  } else if (overlap) {
+    HIPCHK(hipEventRecord(A.ev[(nseg - 1) & 1], rs));
+  }
   for (int s = nseg - 1; s >= 0; s--) {
     const int len = std::min(K, nsteps - s * K);
     if (overlap) HIPCHK(hipStreamWaitEvent(st, A.ev[s & 1], 0));
@@ -2146,6 +2182,9 @@ int gt_adjoint_backward(gt_ctx* c, const double* lx, const double* ly, const dou
   // leave the context at the forward run's final state
   HIPCHK(hipMemcpyAsync(c->b, A.ckpt + (size_t)nseg * f3, sbytes, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  // the recompute of segments 0 .. nseg - 2 refilled the other half only
+  // when nseg <= 2, so then the last segment's states are still in place
+  A.last_kept = nseg <= 2;
   std::vector<double> gsh(c->shell_k2.size(), 0.0);
   for (int n = 0; n < nsteps; n++)
     for (size_t e = 0; e < per; e++) gsh[c->entry_shell[e]] += hdots[per * n + e];
