@@ -122,18 +122,27 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
   const int r1 = r0 + kp.nxp;
   const int kc = kp.kxcut;
   const L lay{g, 0};
+  // the twiddles of the inverse transform loaded with the data (the forward
+  // transform of the same length reads the same entries), at the lengths
+  // whose x passes do so too; the others read the table in every stage
+  constexpr bool TWR = N == 128 || N == 256 || N == 512;
+  typename FI::Twr twr;
   {
     cx<double> in[1][FI::E0], out[1][FI::EL];
     const cx<double>* src = work + g * kp.fs;
-#pragma unroll
-    for (int m = 0; m < FI::M0; m++)
-#pragma unroll
-      for (int r = 0; r < FI::R0; r++) {
-        cx<double> v = {0.0, 0.0};
-        if (FI::in_ok(t, m)) v = c2r_in<double, N>(src + r0, src + r1, FI::in_pos(t, m, r), kc);
-        in[0][m * FI::R0 + r] = v;
-      }
-    FI::run(lay, lds, tw, t, in, out);
+    static_for<0, FI::M0 * FI::R0>([&](auto qc) {
+      constexpr int q = decltype(qc)::value, m = q / FI::R0, r = q % FI::R0;
+      constexpr int C = P * m + r * FI::NR0;  // in_pos(t, m, r) = t + C
+      cx<double> v = {0.0, 0.0};
+      if (FI::in_ok(t, m)) v = c2r_in_at<double, N, P, C>(src + r0, src + r1, t, kc);
+      in[0][q] = v;
+    });
+    if constexpr (TWR) {
+      FI::load_tw(tw, t, twr);
+      FI::run(lay, lds, tw, twr, t, in, out);
+    } else {
+      FI::run(lay, lds, tw, t, in, out);
+    }
     stage_sync<SY>();
 #pragma unroll
     for (int m = 0; m < FI::ML; m++)
@@ -161,7 +170,8 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
     }
   __syncthreads();  // every product read is done before a forward transform writes LDS
   if (SY == SYNC_BLOCK || g < 6) {
-    FF::run(lay, lds, tw, t, fin, fout);
+    if constexpr (TWR) FF::run(lay, lds, tw, twr, t, fin, fout);
+    else FF::run(lay, lds, tw, t, fin, fout);
     stage_sync<SY>();
 #pragma unroll
     for (int m = 0; m < FF::ML; m++)
@@ -171,13 +181,14 @@ __global__ __launch_bounds__(9 * P) void k_xadj(KP kp, cx<double>* work, const c
   }
   __syncthreads();
   if (g < 6) {
-    cx<double>* dst = work + g * kp.fs;
+    cx<double>* d0 = work + g * kp.fs + r0;
+    cx<double>* d1 = work + g * kp.fs + r1;
     for (int k = t; k <= kc; k += P) {
       const int kk = (k == 0) ? 0 : N - k;
       const double h = (k == 0 || k == N / 2) ? 0.5 : 1.0;  // wgt / 2
       const cx<double> Y = lds[g * N + k], Yc = lds[g * N + kk];
-      dst[r0 + k] = {h * (Y.x + Yc.x), h * (Y.y - Yc.y)};
-      dst[r1 + k] = {h * (Y.y + Yc.y), h * (Yc.x - Y.x)};
+      st_row_p<NT_C>(d0, (unsigned)k, cx<double>{h * (Y.x + Yc.x), h * (Y.y - Yc.y)});
+      st_row_p<NT_C>(d1, (unsigned)k, cx<double>{h * (Y.y + Yc.y), h * (Yc.x - Y.x)});
     }
   }
 }
