@@ -83,7 +83,7 @@ enum { EPI_STEP1 = 0, EPI_STEP2 = 1, EPI_NONLIN = 2, EPI_INIT = 3 };
 template <typename W, int N, int T, int P, bool EMK = false, bool PZ = false, bool PAIR = false>
 __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restrict__ b, cx<W>* __restrict__ work,
                                                const cx<W>* __restrict__ tw, int nout) {
-  static_assert(!(PAIR && EMK), "the paired blocks transform b itself");
+  static_assert(!(PAIR && EMK) || sizeof(W) == 8, "the paired EMK blocks keep b in registers (FP64)");
   using LI = LayInterleaved<T, sizeof(cx<W>) == 8 && T == 8>;  // swizzled for 8 pencils of 8 bytes
   using F = Fft<W, N, P, 1, +1, LI>;
   constexpr int R0 = F::R0, RL = F::RL, M0 = F::M0, ML = F::ML;
@@ -276,8 +276,13 @@ __global__ __launch_bounds__(T* P) void k_zinv(KP kp, const cx<double>* __restri
         const int n = F::in_pos(t, m, r);
         const bool use = F::in_ok(t, m) && keep_idx(n, N, K);
         const double fz = F::in_ok(t, m) ? fz_of(qc) : 0.0;
-        const cx<double> vb = use ? lb[qs] : cx<double>{0.0, 0.0};
+        cx<double> vb = use ? lb[qs] : cx<double>{0.0, 0.0};
         const double k2 = fx * fx + fy * fy + fz * fz;
+        if constexpr (EMK) {  // as form() scales it for the first transform
+          const bool k0 = (kx == 0) && (j == 0) && (n == 0);
+          const double e = k0 ? 0.0 : emk_at(kp, kx < kp.nxh ? kx : kp.nxh - 1, j, n, k2);
+          vb = {e * vb.x, e * vb.y};
+        }
         in[0][qs] = {(W)(k2 * vb.x), (W)(k2 * vb.y)};
       } else if constexpr (IN2) {
         in[0][qs] = in2[qs];
@@ -747,9 +752,10 @@ hipError_t launch_zinv_emk(const KP& kp, const void* b, void* work, const void* 
 #define GT_CASE(N)                                                                                     \
   case N: {                                                                                            \
     constexpr int T = pencil_T(N), P = pencil_P(N);                                                    \
-    const int grid = (kp.kxcut + T) / T * kp.ny * 6;                                                   \
-    k_zinv<double, N, T, P, true><<<grid, T * P, N * T * sizeof(cx<double>), s>>>(kp, (const cx<double>*)b, (cx<double>*)work,  \
-                                                          (const cx<double>*)tw, 6);                   \
+    constexpr int NB = GT_ZINV_PAIR ? 3 : 6; /* blocks per column */                                   \
+    const int grid = (kp.kxcut + T) / T * kp.ny * NB;                                                  \
+    k_zinv<double, N, T, P, true, false, GT_ZINV_PAIR != 0><<<grid, T * P, N * T * sizeof(cx<double>), s>>>(       \
+        kp, (const cx<double>*)b, (cx<double>*)work, (const cx<double>*)tw, NB);                         \
     return hipGetLastError();                                                                          \
   }
     GT_FOR_EACH_N(GT_CASE)
