@@ -276,3 +276,35 @@ def test_final_cotangent_is_projected_in_pruned_mode(require_gpu):
         assert np.array_equal(u, v)
         assert not np.any(u[masked])
     assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("dt2_mode", [0, 1])
+def test_segment_layouts_give_one_gradient(require_gpu, dt2_mode):
+    """The same 5-step gradient with checkpoints every 1, 2, 3 and 5 steps
+    (five segments, a ragged last segment of one step, of two steps, and a
+    single segment whose states all come from the forward run): every layout
+    gives the gradient of every other, and a second backward on each run (the
+    last segment recomputed where the recompute of earlier segments refilled
+    its buffer) equals the first bit for bit."""
+    p = Params(nx=32, ny=32, nz=32, nu=2e-3, dt=5e-3, omega=2.0, famp=0.1, k1f=2.0, k2f=3.0, dt2_mode=dt2_mode)
+    say(f"segment layouts, 5 steps, dt2 mode {dt2_mode}")
+    b0 = initial.initial_state(p, seed=6)
+    w = band_w(p, 4.0, 8.0)
+    out = {}
+    with lib.Context(p) as g:
+        for every in (1, 2, 3, 5):
+            g.set_state(b0)
+            g.adjoint_forward(5, 21, every)
+            lam = [2.0 * w * f for f in g.get_state()]
+            g1, a1 = g.adjoint_backward(lam)
+            g2, a2 = g.adjoint_backward(lam)
+            assert all(np.array_equal(u, v) for u, v in zip(g1, g2)), every
+            assert np.array_equal(a1, a2), every
+            out[every] = (g1, a1)
+    g5, a5 = out[5]
+    for every in (1, 2, 3):
+        gg, ga = out[every]
+        e = rel_l2(gg, g5)
+        say(f"  every {every} against one segment: {e:.2e}")
+        assert e < 1e-13
+        assert np.allclose(ga, a5, rtol=1e-13, atol=1e-15 * np.abs(a5).max())
