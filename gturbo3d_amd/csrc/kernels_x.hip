@@ -242,7 +242,7 @@ __global__ __launch_bounds__(3 * 64) void k_xfused3(KP kp, cx<W>* work, const cx
     }
   }
   stage_sync<SY>();
-  // round 1: u_c to row c; Q = -u_{c+1} w_c
+  // exchange 1: u_c to row c; Q = -u_{c+1} w_c
 #pragma unroll
   for (int m = 0; m < FI::ML; m++)
     if (FI::out_ok(t, m))
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(3 * 64) void k_xfused3(KP kp, cx<W>* work, const cx
       pr[0][q] = v;
     }
   __syncthreads();  // every u row has been read
-  // round 2: w_c to row c; P = Q + u_c w_{c+1}
+  // exchange 2: w_c to row c; P = Q + u_c w_{c+1}
 #pragma unroll
   for (int m = 0; m < FI::ML; m++)
     if (FI::out_ok(t, m))
